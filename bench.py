@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Benchmark of the futspace hot path on one MI355X.
+
+Default workload (plain_2048): the 120-frame fly-through over the seeded
+2048^2 map at 1920x1080 with 2048 depth slices; one step renders one frame
+(slice table + render kernel) from maps already resident in HBM.  Prints
+one JSON line with frames/s, Gpix/s, the render kernel's roofline and the
+CPU baseline (the oracle restatement, OpenMP over columns, on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("frames/s & Gpix/s at 1080p/2048² map, plain vs batched; "
+          "fwd+bwd grad step ms")
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="approximate CPU work for the baseline sample")
+    ap.add_argument("--quick", action="store_true", help="few steps, for profiling runs")
+    return ap.parse_args()
+
+
+def cpu_baseline(views, shaded, alt, budget_s):
+    """The oracle painter (the futhark multicore stand-in) on this host's
+    cores over a bounded sample of the same fly-through."""
+    from oracle import fs_oracle as O
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    frames = 0
+    while frames < len(views):
+        O.render_painter_mt(views[frames % len(views)], shaded, alt, threads)
+        frames += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    # single-thread stand-in for `futhark c`, one frame
+    t1 = time.perf_counter()
+    O.render_painter_mt(views[0], shaded, alt, 1)
+    st = time.perf_counter() - t1
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} frames of the 1920x1080 / 2048-slice fly-through "
+                      f"({dt:.1f} s), oracle painter, OpenMP over columns",
+            "single_thread_frames_per_s": 1.0 / st, "cpu_model": cpu_model}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by view or column range, see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.quick:
+        args.steps, args.warmup = min(args.steps, 30), min(args.warmup, 5)
+
+    import numpy as np
+    import torch
+
+    from futspace_amd import Context, scene
+    from futspace_amd import _native
+
+    col, hts = scene.plain_2048_maps()
+    views = scene.plain_2048_views(hts)
+    ctx = Context(device=0)
+    m = ctx.map(col, hts)                  # reference shading: 1024^2 shade map
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream)
+    H, W = views[0].h, views[0].w
+    out = torch.empty((H, W), dtype=torch.int32, device="cuda")
+
+    with torch.cuda.stream(stream):
+        for i in range(args.warmup):
+            ctx.render(m, views[i % len(views)], out)
+        torch.cuda.synchronize()
+        lib = _native.futspace()
+        ctx.check(lib.fsx_profile_enable(ctx.ptr, 1), "fsx_profile_enable")
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            ctx.render(m, views[i % len(views)], out)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        import ctypes
+        tot_ms, n = ctypes.c_double(), ctypes.c_int64()
+        ctx.check(lib.fsx_profile_read(ctx.ptr, ctypes.byref(tot_ms), ctypes.byref(n)),
+                  "fsx_profile_read")
+        ctx.check(lib.fsx_profile_enable(ctx.ptr, 0), "fsx_profile_enable")
+
+    elapsed = t1 - t0
+    fps = args.steps / elapsed
+    kern_ms = tot_ms.value / max(n.value, 1)
+    bytes_per_frame = float(np.mean([scene.algorithmic_bytes(v, 1) for v in views]))
+    achieved = bytes_per_frame / (kern_ms * 1e-3) / 1e9
+    rec = {
+        "metric": METRIC,
+        "value": fps,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded diamond-square 2048^2 map, Catmull-Rom fly-through)",
+        "config": {"workload": "plain_2048: 1920x1080 frames, 2048 depth slices, 2048^2 u8 "
+                               "height + u32 colour map, 120-frame fly-through",
+                   "frame": [H, W], "slices": 2048, "map": 2048},
+        "gpix_per_s": fps * H * W / 1e9,
+        "render_kernel_ms": kern_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_render", "algorithmic_bytes_per_launch": bytes_per_frame},
+    }
+    if not args.no_cpu:
+        from oracle import fs_oracle as O
+        shaded, alt = O.update_map(col, hts)
+        rec["cpu_baseline"] = cpu_baseline(views, shaded, alt, args.cpu_seconds)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

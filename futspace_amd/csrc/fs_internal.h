@@ -1,0 +1,63 @@
+/*
+ * fs_internal.h -- interface between the C host library (fs_host.c) and
+ * the HIP kernel launchers (fs_kernels.hip).  Not installed; no torch or
+ * HIP C++ types cross it.
+ */
+#ifndef FS_INTERNAL_H
+#define FS_INTERNAL_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device view of a resident map pair.
+ *   hq: per texel (r,c) the bilinear quad of masked heights
+ *       h(r,c), h(r,c+1), h(r+1,c), h(r+1,c+1) (indices wrap), packed as
+ *       4 x u8 in a u32 (hbits 8) or 4 x u16 in a uint2 (hbits 16);
+ *   cq: the same quad of shaded colours, one uint4 per shade texel. */
+typedef struct {
+    const void *hq;
+    int32_t hrows, hcols, hbits;
+    const void *cq;
+    int32_t crows, ccols;
+} fsk_map;
+
+/* Per-frame scalars, computed on the host exactly as `futhark c` does:
+ *   lx = Lx*z, ly = Ly*z, rx = Rx*z, ry = Ry*z  (fut/voxel_renderer.fut:47-50)
+ * with Lx = (-cos a) - sin a*fov, Ly = sin a - cos a*fov,
+ *      Rx = cos a - sin a*fov,   Ry = (-sin a) - cos a*fov. */
+typedef struct {
+    float cam_x, cam_y, cam_h, horizon;
+    float Lx, Ly, Rx, Ry;
+    float d, z0, scale, fw;
+    int32_t n, rows, cols;
+    uint32_t sky;
+} fsk_frame;
+
+/* Slice table: n records of {start_x, start_y, dx, dy} then n inv_z. */
+int fsk_slices(const fsk_frame *f, float *d_tab, hipStream_t s);
+
+/* Plain render of one frame into out[rows][pitch]. */
+int fsk_render(const fsk_map *m, const fsk_frame *f, const float *d_tab,
+               uint32_t *d_out, int64_t pitch, hipStream_t s);
+/* LDS bytes the render kernel needs for a frame of `rows` rows. */
+size_t fsk_render_lds_bytes(int32_t rows);
+int32_t fsk_render_max_rows(void);
+
+/* Map preparation. */
+int fsk_height_quads(const int32_t *d_alt, int32_t rows, int32_t cols, int32_t hbits,
+                     void *d_hq, uint16_t *d_hplain, hipStream_t s);
+int fsk_shade(const uint16_t *d_hplain, int32_t hrows, int32_t hcols,
+              const uint32_t *d_color, int32_t crows, int32_t ccols,
+              float sun0, float sun1, float sun2, int32_t size,
+              uint32_t *d_shaded, hipStream_t s);
+int fsk_color_quads(const uint32_t *d_col, int32_t rows, int32_t cols, void *d_cq,
+                    hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,192 @@
+/*
+ * libfutspace.h -- C-ABI of the MI355X (gfx950) futspace renderer.
+ *
+ * Part 1 declares exactly the symbols `futhark opencl --library
+ * fut/interactive_entrypoints.fut` generates for the reference
+ * (Makefile:24-25), with the same prototypes, so the reference host
+ * c/interactive.c and the lys frame loop link against this library
+ * unchanged (the header is passed as PROGHEADER, Makefile:19).
+ * Part 2 declares extension entries with no reference counterpart:
+ * explicit-camera, batched and differentiable renders over resident maps,
+ * used by tests, benchmarks and the PyTorch harness.
+ *
+ * Conventions (those of the generated library):
+ *  - every entry returns 0 on success; on failure it returns nonzero and
+ *    leaves a message that futhark_context_get_error() hands over (a
+ *    malloc'd string the caller frees);
+ *  - out-parameters come first; every returned array or state is a new
+ *    object owned by the caller and freed with futhark_free_*; inputs are
+ *    borrowed;
+ *  - one context per thread; entries are not reentrant on a context.
+ * All device work of a context runs on one HIP stream.
+ */
+#ifndef LIBFUTSPACE_H
+#define LIBFUTSPACE_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include <stdbool.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ==================================================================== */
+/* Part 1: the generated library's interface                             */
+/* ==================================================================== */
+
+/* ---- Initialisation (generated for every backend) ---- */
+struct futhark_context_config;
+struct futhark_context_config *futhark_context_config_new(void);
+void futhark_context_config_free(struct futhark_context_config *cfg);
+void futhark_context_config_set_debugging(struct futhark_context_config *cfg, int flag);
+void futhark_context_config_set_profiling(struct futhark_context_config *cfg, int flag);
+void futhark_context_config_set_logging(struct futhark_context_config *cfg, int flag);
+/* GPU backends: select a device by "#<index>" or by a substring of its
+ * name (called by lys' context setup when -d is given, c/interactive.c:127-129). */
+void futhark_context_config_set_device(struct futhark_context_config *cfg, const char *s);
+/* OpenCL backend: list devices (c/interactive.c:130-132 -i); here it lists
+ * the HIP devices on stderr and keeps the current choice. */
+void futhark_context_config_select_device_interactively(struct futhark_context_config *cfg);
+
+struct futhark_context;
+struct futhark_context *futhark_context_new(struct futhark_context_config *cfg);
+void futhark_context_free(struct futhark_context *ctx);
+int futhark_context_sync(struct futhark_context *ctx);
+char *futhark_context_get_error(struct futhark_context *ctx);
+int futhark_context_clear_caches(struct futhark_context *ctx);
+void futhark_context_pause_profiling(struct futhark_context *ctx);
+void futhark_context_unpause_profiling(struct futhark_context *ctx);
+void futhark_debugging_report(struct futhark_context *ctx);
+
+/* ---- Arrays (c/interactive.c:50-56) ---- */
+struct futhark_i32_2d;
+struct futhark_i32_2d *futhark_new_i32_2d(struct futhark_context *ctx, const int32_t *data,
+                                          int64_t dim0, int64_t dim1);
+int futhark_free_i32_2d(struct futhark_context *ctx, struct futhark_i32_2d *arr);
+int futhark_values_i32_2d(struct futhark_context *ctx, struct futhark_i32_2d *arr, int32_t *data);
+const int64_t *futhark_shape_i32_2d(struct futhark_context *ctx, struct futhark_i32_2d *arr);
+
+struct futhark_u32_2d;
+struct futhark_u32_2d *futhark_new_u32_2d(struct futhark_context *ctx, const uint32_t *data,
+                                          int64_t dim0, int64_t dim1);
+int futhark_free_u32_2d(struct futhark_context *ctx, struct futhark_u32_2d *arr);
+int futhark_values_u32_2d(struct futhark_context *ctx, struct futhark_u32_2d *arr, uint32_t *data);
+const int64_t *futhark_shape_u32_2d(struct futhark_context *ctx, struct futhark_u32_2d *arr);
+
+/* ---- Opaque values: `state`, fut/interactive_entrypoints.fut:4 ---- */
+struct futhark_opaque_state;
+int futhark_free_opaque_state(struct futhark_context *ctx, struct futhark_opaque_state *obj);
+
+/* ---- Entry points, fut/interactive_entrypoints.fut:6-32 ---- */
+/* entry resize (h: i32) (w: i32) (s: state): state        (:6-7)  */
+int futhark_entry_resize(struct futhark_context *ctx, struct futhark_opaque_state **out0,
+                         const int32_t in0, const int32_t in1,
+                         const struct futhark_opaque_state *in2);
+/* entry key (e: i32) (key: i32) (s: state): state          (:9-11) */
+int futhark_entry_key(struct futhark_context *ctx, struct futhark_opaque_state **out0,
+                      const int32_t in0, const int32_t in1,
+                      const struct futhark_opaque_state *in2);
+/* entry mouse (buttons: i32) (x: i32) (y: i32) (s: state)  (:13-14) */
+int futhark_entry_mouse(struct futhark_context *ctx, struct futhark_opaque_state **out0,
+                        const int32_t in0, const int32_t in1, const int32_t in2,
+                        const struct futhark_opaque_state *in3);
+/* entry wheel (dx: i32) (dy: i32) (s: state)                (:16-17) */
+int futhark_entry_wheel(struct futhark_context *ctx, struct futhark_opaque_state **out0,
+                        const int32_t in0, const int32_t in1,
+                        const struct futhark_opaque_state *in2);
+/* entry render (s: state): [h][w]u32 -- the hot path        (:19-20) */
+int futhark_entry_render(struct futhark_context *ctx, struct futhark_u32_2d **out0,
+                         const struct futhark_opaque_state *in0);
+/* entry init (seed: u32): state                             (:22-23) */
+int futhark_entry_init(struct futhark_context *ctx, struct futhark_opaque_state **out0,
+                       const uint32_t in0);
+/* entry text_content (s: state): 9 x f32                    (:25-26) */
+int futhark_entry_text_content(struct futhark_context *ctx, float *out0, float *out1,
+                               float *out2, float *out3, float *out4, float *out5,
+                               float *out6, float *out7, float *out8,
+                               const struct futhark_opaque_state *in0);
+/* entry step (td: f32) (s: state): state                    (:28-29) */
+int futhark_entry_step(struct futhark_context *ctx, struct futhark_opaque_state **out0,
+                       const float in0, const struct futhark_opaque_state *in1);
+/* entry update_map [h][w] (color_map: [h][w]u32) (height_map: [h][w]i32)
+ *   (s: state): state                                        (:31-32) */
+int futhark_entry_update_map(struct futhark_context *ctx, struct futhark_opaque_state **out0,
+                             const struct futhark_u32_2d *in0,
+                             const struct futhark_i32_2d *in1,
+                             const struct futhark_opaque_state *in2);
+
+/* ==================================================================== */
+/* Part 2: extension entries (no counterpart in the reference)           */
+/* ==================================================================== */
+
+/* One view: the reference camera record (fut/voxel_renderer.fut:5-12),
+ * the frame size, and the depth/projection parameters the reference
+ * hard-codes -- d (invz_param1, fut/voxel_renderer.fut:104), z0
+ * (invz_param2, :103), scale (magic_number, :217; <= 0 means f32(w/2)),
+ * n_slices (<= 0 means get_zs' count from distance, :28-34).  `map`
+ * indexes the map list of a batched render. */
+struct fsx_view {
+    float x, y, height, angle, horizon, distance, fov;
+    uint32_t sky_color;
+    int32_t h, w;
+    float d, z0, scale;
+    int32_t n_slices;
+    int32_t map;
+    int32_t reserved;
+};
+
+/* A device-resident map pair: heights masked to `height_bits` (8 is the
+ * reference's `& 0xFF`, fut/interactive.fut:189; 16 keeps u16 heights)
+ * and the shaded colour map of generate_shadowmap_accumulated for the
+ * given sun (fut/effects.fut:108-125) on a shade_size^2 grid (0 selects
+ * the reference's fixed 1024). */
+struct fsx_map;
+int fsx_map_new(struct futhark_context *ctx, struct fsx_map **out,
+                const uint32_t *color, const int32_t *height, int64_t rows, int64_t cols,
+                int32_t height_bits, int32_t shade_size, float sun_height, float sun_ang);
+int fsx_map_free(struct futhark_context *ctx, struct fsx_map *map);
+/* Copy the shaded colour map (shade_size^2 u32) to host memory. */
+int fsx_map_shaded(struct futhark_context *ctx, const struct fsx_map *map, uint32_t *host_out);
+int32_t fsx_map_shade_size(const struct fsx_map *map);
+
+/* Render one view into device memory `dev_out` ([h][w] u32, row pitch
+ * `pitch` elements; 0 means w).  Asynchronous on the context stream. */
+int fsx_render(struct futhark_context *ctx, const struct fsx_map *map,
+               const struct fsx_view *view, uint32_t *dev_out, int64_t pitch);
+/* Same, into host memory (synchronous). */
+int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,
+                    const struct fsx_view *view, uint32_t *host_out);
+
+/* Batched render: views[i] draws maps[views[i].map] into dev_out +
+ * offsets[i] (u32 elements, row pitch views[i].w).  One persistent
+ * launch covers all views. */
+int fsx_render_batch(struct futhark_context *ctx, const struct fsx_map *const *maps,
+                     int32_t n_maps, const struct fsx_view *views, int32_t n_views,
+                     uint32_t *dev_out, const int64_t *offsets);
+
+/* Use `stream` (a hipStream_t, or NULL for the context's own stream) for
+ * all later device work of the context. */
+int fsx_context_set_stream(struct futhark_context *ctx, void *stream);
+/* HIP device index the context runs on. */
+int fsx_context_device(struct futhark_context *ctx);
+
+/* Kernel timing: while enabled, every render-kernel launch is bracketed
+ * by HIP events on its own stream (up to 4096 launches are kept).
+ * fsx_profile_read synchronises, returns the summed kernel time in ms and
+ * the number of launches, and clears the record. */
+int fsx_profile_enable(struct futhark_context *ctx, int enable);
+int fsx_profile_read(struct futhark_context *ctx, double *total_ms, int64_t *launches);
+
+/* The sky colour `step` sets for a sun height: argb.scale 0xFF9090e0
+ * sun_height (fut/interactive.fut:163). */
+uint32_t fsx_sky(float sun_height);
+
+/* Version string of this library. */
+const char *fsx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""Bit-exact parity of the HIP renderer with the CPU oracle (needs a GPU).
+
+Every frame, shaded colour map and state-machine output must equal the
+oracle's bit for bit: the renderer is integer-exact by construction (f32
+in the reference's operation order, no contraction, x86 cast semantics).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from futspace_amd import View, scene, sky_color
+from oracle import fs_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _diff_report(got, want):
+    bad = np.argwhere(got != want)
+    if len(bad) == 0:
+        return ""
+    r, c = bad[0]
+    return (f"{len(bad)} of {got.size} pixels differ; first at row {r} col {c}: "
+            f"got {int(got[r, c]):#010x} want {int(want[r, c]):#010x}")
+
+
+def _check(got, want):
+    assert got.shape == want.shape
+    msg = _diff_report(got, want)
+    assert not msg, msg
+
+
+@pytest.fixture(scope="module")
+def small_scene():
+    col, hts = scene.scene_maps(11, 128)
+    return col, hts
+
+
+def test_shade_matches_oracle(ctx, small_scene):
+    col, hts = small_scene
+    m = ctx.map(col, hts)
+    shaded, _ = O.update_map(col, hts)
+    _check(m.shaded(), shaded)
+
+
+@pytest.mark.parametrize("sun_h,sun_a", [(0.1, 0.1), (0.6, 2.0), (-0.3, 0.7), (1.2, -2.5)])
+def test_shade_sun_angles(ctx, small_scene, sun_h, sun_a):
+    col, hts = small_scene
+    m = ctx.map(col, hts, sun_height=sun_h, sun_ang=sun_a)
+    shaded, _ = O.update_map(col, hts, sun_h, sun_a)
+    _check(m.shaded(), shaded)
+
+
+SMALL_VIEWS = [
+    # x, y, height, angle, horizon, distance, fov, h, w, d, z0, scale, n
+    (64.3, 100.7, 260.0, 0.4, 60.0, 300.0, 1.2, 96, 128, 0.01, 0.5, 0.0, 0),
+    (64.0, 64.0, 200.0, 0.0, 50.0, 200.0, 1.2, 64, 80, 0.01, 0.5, 0.0, 0),      # yaw 0: integer x
+    (10.5, 3.25, 300.0, 1.5707964, 90.0, 400.0, 0.8, 100, 77, 0.005, 1.0, 120.0, 0),
+    (-40.2, 900.1, 180.0, 3.0, 20.0, 250.0, 1.5, 33, 31, 0.02, 0.25, 0.0, 0),
+    (5.0, 5.0, 140.0, -2.2, 150.0, 500.0, 1.2, 120, 64, 0.001, 0.0, 0.0, 0),    # z0 = 0
+    (70.0, 30.0, 400.0, 0.9, -50.0, 600.0, 1.1, 64, 64, 0.0, 2.0, 0.0, 300),     # d = 0, explicit n
+    (33.3, 77.7, 250.0, 5.0, 300.0, 700.0, 1.2, 257, 129, 0.003, 0.4, 64.0, 0),
+]
+
+
+@pytest.mark.parametrize("vi", range(len(SMALL_VIEWS)))
+def test_small_views_bitexact(ctx, small_scene, vi):
+    col, hts = small_scene
+    x, y, hh, a, hor, dist, fov, h, w, d, z0, sc, n = SMALL_VIEWS[vi]
+    v = View(x=x, y=y, height=hh, angle=a, horizon=hor, distance=dist, fov=fov,
+             sky_color=sky_color(0.1), h=h, w=w, d=d, z0=z0, scale=sc, n_slices=n)
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 1), (3, 5), (96, 80), (128, 64)])
+def test_non_power_of_two_maps(ctx, rows, cols):
+    hts = (np.arange(rows * cols, dtype=np.int32).reshape(rows, cols) * 37) % 256
+    col = (0xFF000000 | (np.arange(rows * cols, dtype=np.uint32).reshape(rows, cols) * 2654435761
+                         % 0xFFFFFF)).astype(np.uint32)
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    _check(m.shaded(), shaded)
+    v = View(x=0.37, y=0.61, height=300.0, angle=2.2, horizon=200.0, distance=400.0, fov=1.2,
+             sky_color=sky_color(0.1), h=70, w=90, d=0.004, z0=0.3, scale=0.0)
+    _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+
+
+def test_zero_colour_texels_are_transparent(ctx):
+    """Colour-0 texels are `empty` for fill_vline and sky
+    (fut/voxel_renderer.fut:78-81, :248)."""
+    col, hts = scene.scene_maps(12, 64)
+    col = col.copy()
+    col[::3, :] = 0
+    col[:, ::5] = 0
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    v = View(x=20.5, y=40.2, height=220.0, angle=0.3, horizon=80.0, distance=200.0, fov=1.2,
+             sky_color=sky_color(0.1), h=128, w=96, d=0.01, z0=0.5)
+    _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+
+
+def test_sixteen_bit_heights(ctx):
+    hts = scene.heightmap(13, 128, 0.6, 600)
+    col = scene.colormap(13, hts, 600)
+    m = ctx.map(col, hts, height_bits=16, shade_size=128)
+    shaded, alt = O.update_map(col, hts, size=128, bits=16)
+    _check(m.shaded(), shaded)
+    v = View(x=50.0, y=90.0, height=700.0, angle=1.0, horizon=120.0, distance=1000.0, fov=1.2,
+             sky_color=sky_color(0.1), h=144, w=160, d=0.002, z0=0.5)
+    _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+
+
+def test_plain_1024_bitexact(ctx):
+    col, hts = scene.plain_1024_maps()
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    _check(m.shaded(), shaded)
+    v = scene.plain_1024_view()
+    _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+
+
+@pytest.fixture(scope="module")
+def plain_2048(ctx):
+    col, hts = scene.plain_2048_maps()
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    return m, shaded, alt, scene.plain_2048_views(hts)
+
+
+@pytest.mark.parametrize("frame", [0, 8, 31, 64, 97, 119])
+def test_plain_2048_flythrough_frames(ctx, plain_2048, frame):
+    m, shaded, alt, views = plain_2048
+    v = views[frame]
+    _check(ctx.render(m, v), O.render_painter_mt(v, shaded, alt))
+
+
+def test_render_into_torch_tensor(ctx, small_scene):
+    torch = pytest.importorskip("torch")
+    col, hts = small_scene
+    m = ctx.map(col, hts)
+    v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=60.0, distance=300.0, fov=1.2,
+             sky_color=sky_color(0.1), h=96, w=128, d=0.01, z0=0.5)
+    out = torch.zeros((96, 136), dtype=torch.int32, device="cuda")
+    ctx.set_stream(torch.cuda.current_stream())
+    ctx.render(m, v, out)
+    torch.cuda.synchronize()
+    ctx.set_stream(None)
+    got = out.cpu().numpy().view(np.uint32)
+    shaded, alt = O.update_map(col, hts)
+    _check(got[:, :128].copy(), O.render_off(v, shaded, alt))
+    assert not got[:, 128:].any(), "render wrote past the frame width"
+
+
+def test_batch_matches_single_views(ctx):
+    torch = pytest.importorskip("torch")
+    maps_np = [scene.scene_maps(s, n) for s, n in ((3, 256), (4, 128))]
+    maps = [ctx.map(c, h) for c, h in maps_np]
+    views = []
+    for i in range(9):
+        views.append(View(x=17.0 * i + 0.3, y=11.0 * i + 0.7, height=300.0, angle=0.7 * i,
+                          horizon=100.0, distance=500.0, fov=1.2, sky_color=sky_color(0.1),
+                          h=48 + 16 * (i % 3), w=64 + 8 * i, d=0.004, z0=0.5, map=i % 2))
+    offs, total = scene.batch_offsets(views)
+    out = torch.zeros(total, dtype=torch.int32, device="cuda")
+    ctx.set_stream(torch.cuda.current_stream())
+    ctx.render_batch(maps, views, out, offs)
+    torch.cuda.synchronize()
+    ctx.set_stream(None)
+    flat = out.cpu().numpy().view(np.uint32)
+    for v, o in zip(views, offs):
+        got = flat[o:o + v.h * v.w].reshape(v.h, v.w)
+        _check(got, ctx.render(maps[v.map], v))
