@@ -30,7 +30,20 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--quick", action="store_true", help="few steps, for profiling runs")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 64 MiB device copy (known bytes) to calibrate HBM counters")
     return ap.parse_args()
+
+
+def calib_copy():
+    """A plain device-to-device copy of known size: 64 MiB read + 64 MiB
+    written per launch, the yardstick for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    src = torch.ones(16 * 1024 * 1024, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def cpu_baseline(views, shaded, alt, budget_s):
@@ -134,6 +147,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_render", "algorithmic_bytes_per_launch": bytes_per_frame},
     }
+    if args.calib_copy:
+        calib_copy()
     if not args.no_cpu:
         from oracle import fs_oracle as O
         shaded, alt = O.update_map(col, hts)

@@ -105,6 +105,7 @@ FUTSPACE_SIGNATURES = {
     "fsx_context_device": (ctypes.c_int, [P]),
     "fsx_version": (ctypes.c_char_p, []),
     "fsx_sky": (U32, [F32]),
+    "fsx_selftest": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
     "fsx_profile_enable": (ctypes.c_int, [P, ctypes.c_int]),
     "fsx_profile_read": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I64)]),
 }

@@ -349,6 +349,36 @@ const char *fsx_version(void)
     return FS_VERSION;
 }
 
+int fsx_selftest(struct futhark_context *ctx, int which, int64_t *mismatches,
+                 int64_t *first_bad)
+{
+    if (!ctx || !mismatches)
+        return 1;
+    unsigned long long *d = NULL, h[2] = {0, 0};
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    if (dev_alloc(ctx, (void **)&d, sizeof h))
+        return 1;
+    /* count = 0, smallest failing input = all ones */
+    hipError_t e = hipMemsetAsync(d, 0, sizeof h[0], ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(d + 1, 0xFF, sizeof h[1], ctx->stream);
+    if (e == hipSuccess)
+        e = (hipError_t)fsk_selftest(which, d, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    dev_release(ctx, d, sizeof h);
+    if (e != hipSuccess) {
+        set_error(ctx, "fsx_selftest(%d): %s", which, hipGetErrorString(e));
+        return 1;
+    }
+    *mismatches = (int64_t)h[0];
+    if (first_bad)
+        *first_bad = h[0] ? (int64_t)h[1] : -1;
+    return 0;
+}
+
 int fsx_profile_enable(struct futhark_context *ctx, int enable)
 {
     if (!ctx)
@@ -597,6 +627,10 @@ static struct fs_terrain *terrain_new(struct futhark_context *ctx, const uint32_
                                       const int32_t *d_alt, int32_t rows, int32_t cols,
                                       int32_t hbits)
 {
+    if ((int64_t)rows * (int64_t)cols > ((int64_t)1 << 28)) {
+        set_error(ctx, "map of %d x %d texels exceeds the supported 2^28", rows, cols);
+        return NULL;
+    }
     struct fs_terrain *t = (struct fs_terrain *)calloc(1, sizeof *t);
     if (!t)
         return NULL;
@@ -787,8 +821,6 @@ static int render_into(struct futhark_context *ctx, const struct fs_terrain *t,
         return 0;
     fsk_frame f;
     frame_of_view(v, &f);
-    if (ensure_tab(ctx, f.n))
-        return 1;
     fsk_map m;
     m.hq = t->d_hq;
     m.hrows = t->rows;
@@ -798,9 +830,8 @@ static int render_into(struct futhark_context *ctx, const struct fs_terrain *t,
     m.crows = s->rows;
     m.ccols = s->cols;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    HIPCHECK(ctx, fsk_slices(&f, ctx->d_tab, ctx->stream));
     int pe = prof_begin(ctx);
-    HIPCHECK(ctx, fsk_render(&m, &f, ctx->d_tab, d_out, pitch > 0 ? pitch : v->w, ctx->stream));
+    HIPCHECK(ctx, fsk_render(&m, &f, d_out, pitch > 0 ? pitch : v->w, ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }

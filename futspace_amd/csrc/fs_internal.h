@@ -40,12 +40,17 @@ typedef struct {
 /* Slice table: n records of {start_x, start_y, dx, dy} then n inv_z. */
 int fsk_slices(const fsk_frame *f, float *d_tab, hipStream_t s);
 
-/* Plain render of one frame into out[rows][pitch]. */
-int fsk_render(const fsk_map *m, const fsk_frame *f, const float *d_tab,
-               uint32_t *d_out, int64_t pitch, hipStream_t s);
+/* Plain render of one frame into out[rows][pitch]; the kernel builds its
+ * own slice table per depth chunk. */
+int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
+               hipStream_t s);
 /* LDS bytes the render kernel needs for a frame of `rows` rows. */
 size_t fsk_render_lds_bytes(int32_t rows);
 int32_t fsk_render_max_rows(void);
+
+/* Device self-test `which` (0-3) of the arithmetic shortcuts; adds the
+ * number of mismatching inputs to *d_bad. */
+int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s);
 
 /* Map preparation. */
 int fsk_height_quads(const int32_t *d_alt, int32_t rows, int32_t cols, int32_t hbits,

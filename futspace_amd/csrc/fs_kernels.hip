@@ -19,6 +19,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "fs_internal.h"
 
@@ -50,6 +51,15 @@ __device__ __forceinline__ int32_t i32_f32_x86(float v)
 __device__ __forceinline__ int32_t screen_row(float rel)
 {
     return (rel < 2147483648.0f) ? max(0, (int32_t)fmaxf(rel, -1.0f)) : 0;
+}
+
+/* min(H, screen_row(rel)) with H = rows as float: truncating the median of
+ * (rel, 0, H) equals clamping the truncation for every rel below 2^31;
+ * rel >= 2^31 and NaN take the select's 0. */
+__device__ __forceinline__ int32_t screen_row_clamped(float rel, float Hf)
+{
+    int32_t r = (int32_t)__builtin_amdgcn_fmed3f(rel, 0.0f, Hf);
+    return (rel < 2147483648.0f) ? r : 0;
 }
 
 /* Futhark's signed `%` (floor-mod). */
@@ -84,9 +94,35 @@ __device__ __forceinline__ float unit8(uint32_t b)
 /* u32.f32 (clamp_channel v * 255); clamp passes NaN, u32 of NaN is 0. */
 __device__ __forceinline__ uint32_t chan_u8(float v)
 {
-    float c = (v < 0.0f) ? 0.0f : ((v > 1.0f) ? 1.0f : v);
-    float s = c * 255.0f;
-    return (s == s) ? (uint32_t)s : 0u;
+    /* the clamp is a median of three; a NaN v is caught by the final
+     * select, so whatever the median does with it is discarded */
+    float c = __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f);
+    uint32_t u = (uint32_t)(c * 255.0f);
+    return (v == v) ? u : 0u;
+}
+
+/* Correctly rounded sqrt for t in [0, 1] or NaN: v_sqrt_f32 is within one
+ * ulp, and the two FMA residuals pick the rounded root among its
+ * neighbours.  Inputs are never negative or infinite here, which drops the
+ * general sequence's special cases; inputs below 2^-96 (only reachable
+ * within ~2^-90 of the map origin) take the general sqrt on a wave-uniform
+ * branch, which skips it everywhere else.  The device self-test compares
+ * this with the general sqrt on every f32 in [0, 1]. */
+__device__ __forceinline__ float sqrt_unit(float t)
+{
+    float s = __builtin_amdgcn_sqrtf(t);
+    float sm = __int_as_float(__float_as_int(s) - 1);
+    float sp = __int_as_float(__float_as_int(s) + 1);
+    float rm = __builtin_fmaf(-sm, s, t);
+    float rp = __builtin_fmaf(-sp, s, t);
+    s = (rm <= 0.0f) ? sm : s;
+    s = (rp > 0.0f) ? sp : s;
+    bool tiny = (t < 0x1p-96f) && (t > 0.0f);
+    if (__ballot(tiny)) {
+        if (tiny)
+            s = __builtin_sqrtf(t);
+    }
+    return s;
 }
 
 __device__ __forceinline__ uint32_t from_rgba(float r, float g, float b, float a)
@@ -130,16 +166,17 @@ __device__ __forceinline__ float height_bilinear(const fsk_map &m, float x, floa
     float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
     int32_t ix = wrap<HP2>(idx_of(fx), m.hcols);
     int32_t iy = wrap<HP2>(idx_of(fy), m.hrows);
-    size_t t = (size_t)iy * (size_t)m.hcols + (size_t)ix;
+    /* maps are at most 2^28 texels (host-checked), so 32-bit offsets */
+    uint32_t t = (uint32_t)iy * (uint32_t)m.hcols + (uint32_t)ix;
     float h00, h01, h10, h11;
     if (HBITS == 8) {
-        uint32_t q = ((const uint32_t *)m.hq)[t];
+        uint32_t q = *(const uint32_t *)((const char *)m.hq + t * 4u);
         h00 = (float)(q & 255u);
         h01 = (float)((q >> 8) & 255u);
         h10 = (float)((q >> 16) & 255u);
         h11 = (float)(q >> 24);
     } else {
-        uint2 q = ((const uint2 *)m.hq)[t];
+        uint2 q = *(const uint2 *)((const char *)m.hq + t * 8u);
         h00 = (float)(q.x & 65535u);
         h01 = (float)(q.x >> 16);
         h10 = (float)(q.y & 65535u);
@@ -166,40 +203,61 @@ __device__ __forceinline__ uint32_t color_bilinear(const fsk_map &m, float x, fl
 }
 
 /* ------------------------------------------------------------------ */
-/* Slice table: get_zs terms (fut/voxel_renderer.fut:34), get_h_line   */
-/* (:43-60) and inv_z (:217), one thread per slice                     */
+/* Slice parameters: get_zs term (fut/voxel_renderer.fut:34), get_h_line */
+/* (:43-60) and inv_z (:217) of slice k (0-based)                      */
 /* ------------------------------------------------------------------ */
+
+struct Slice {
+    float sx, sy, dx, dy, iz;
+};
+
+__device__ __forceinline__ Slice slice_params(const fsk_frame &f, int32_t k)
+{
+    float i = (float)(k + 1);
+    float z = (i / 2.0f) * ((2.0f * f.z0) + ((i - 1.0f) * f.d));
+    float left_x = f.Lx * z, left_y = f.Ly * z;
+    float right_x = f.Rx * z, right_y = f.Ry * z;
+    Slice s;
+    s.dx = (right_x - left_x) / f.fw;
+    s.dy = (right_y - left_y) / f.fw;
+    s.sx = left_x + f.cam_x;
+    s.sy = left_y + f.cam_y;
+    s.iz = (1.0f / z) * f.scale;
+    return s;
+}
 
 __global__ void k_slices(fsk_frame f, float4 *__restrict__ lines, float *__restrict__ invz)
 {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= f.n)
         return;
-    float i = (float)(k + 1);
-    float z = (i / 2.0f) * ((2.0f * f.z0) + ((i - 1.0f) * f.d));
-    float left_x = f.Lx * z, left_y = f.Ly * z;
-    float right_x = f.Rx * z, right_y = f.Ry * z;
-    float dx = (right_x - left_x) / f.fw;
-    float dy = (right_y - left_y) / f.fw;
-    lines[k] = make_float4(left_x + f.cam_x, left_y + f.cam_y, dx, dy);
-    invz[k] = (1.0f / z) * f.scale;
+    Slice s = slice_params(f, k);
+    lines[k] = make_float4(s.sx, s.sy, s.dx, s.dy);
+    invz[k] = s.iz;
 }
 
 /* ------------------------------------------------------------------ */
 /* The render kernel                                                   */
 /* ------------------------------------------------------------------ */
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-constexpr int kTC = 4;                  /* columns per workgroup */
-constexpr int kSG = kThreads / kTC;     /* slice groups per workgroup */
-constexpr int kKR = 8;                  /* slices per lane per chunk */
-constexpr int kChunk = kSG * kKR;       /* slices per chunk */
-constexpr int kMaxRows = 4096;          /* queue entries keep the row in 16 bits */
+constexpr int kThreads = 1024;
+constexpr int kWaves = kThreads / 64;            /* 16 */
+constexpr int kChunk = 2048;                     /* slices per chunk, at most */
+constexpr int kTabStride = kChunk + kChunk / 16; /* skewed slice-table entries */
+constexpr int kMaxRows = 4096;                   /* queue entries keep the row in 12 bits */
 
-/* blockIdx -> column tile, keeping adjacent tiles on one XCD (workgroups
- * are dispatched round-robin over the 8 XCDs) so neighbouring stripes share
- * an L2 for height texels and for merging the 16-byte row writes. */
+/* Slice s of a chunk lives at s + s/16: the 16 slice groups of a wave
+ * read entries 8 apart, which the skew spreads so that a float4 read
+ * needs only the two passes its 256 bytes take anyway. */
+__device__ __forceinline__ int32_t skew(int32_t s)
+{
+    return s + (s >> 4);
+}
+
+/* blockIdx -> column tile.  Workgroups are dispatched round-robin over
+ * the 8 XCDs (workgroup i runs on XCD i mod 8), so tile ranges are dealt
+ * out per XCD: neighbouring stripes share one L2 for their height texels
+ * and for merging their partial-line row writes. */
 __device__ __forceinline__ int32_t xcd_tile(int32_t b, int32_t n)
 {
     const int32_t X = 8;
@@ -209,150 +267,420 @@ __device__ __forceinline__ int32_t xcd_tile(int32_t b, int32_t n)
 
 struct RenderArgs {
     fsk_map m;
-    const float4 *lines;
-    const float *invz;
+    fsk_frame f;
     uint32_t *out;
     int64_t pitch;
-    int32_t n, rows, cols, ntiles;
-    float cam_h, horizon;
-    uint32_t sky;
+    int32_t ntiles;
 };
 
-template <int HBITS, bool HP2, bool CP2>
-__global__ __launch_bounds__(kThreads) void k_render(RenderArgs a)
+#ifndef FS_SQ_LUT
+#define FS_SQ_LUT 1
+#endif
+
+/* RN(RN(b/255)^2) for every byte: the squared channel of to_rgba. */
+__device__ __forceinline__ void fill_square_lut(float *lut, int32_t tid)
 {
+    if (tid < 256) {
+        float u = unit8((uint32_t)tid);
+        lut[tid] = u * u;
+    }
+}
+
+__device__ __forceinline__ float sq8(const float *lut, uint32_t b)
+{
+#if FS_SQ_LUT
+    return lut[b];
+#else
+    (void)lut;
+    float u = unit8(b);
+    return u * u;
+#endif
+}
+
+/* One channel of argb.mix with the weights already normalised:
+ * u32.f32(clamp(sqrt(m1' * c1^2 + m2' * c2^2)) * 255). */
+__device__ __forceinline__ uint32_t mix_channel(float m1n, float s1, float m2n, float s2)
+{
+    return chan_u8(sqrt_unit((m1n * s1) + (m2n * s2)));
+}
+
+/* The common case of png_color_filtered: |x|, |y| >= 1 and four opaque
+ * texels.  Then the weights (c - x), (x - f) are exact and sum to exactly
+ * 1 (or are both 0 at an integer coordinate), so every normalisation is
+ * the identity, every sqrt argument lies in {0} U [2^-39, 1] (weights are
+ * multiples of 2^-23, squares at least 255^-2), the clamp never acts and
+ * no NaN arises; alpha is 255 unless a weight sum is 0. */
+__device__ __forceinline__ uint32_t mix_channel_exact(float m1, float s1, float m2, float s2)
+{
+    float t = (m1 * s1) + (m2 * s2);
+    float s = __builtin_amdgcn_sqrtf(t);
+    float sm = __int_as_float(__float_as_int(s) - 1);
+    float sp = __int_as_float(__float_as_int(s) + 1);
+    float rm = __builtin_fmaf(-sm, s, t);
+    float rp = __builtin_fmaf(-sp, s, t);
+    s = (rm <= 0.0f) ? sm : s;
+    s = (rp > 0.0f) ? sp : s;
+    return (uint32_t)(s * 255.0f);
+}
+
+__device__ __forceinline__ uint32_t color_fast(uint4 q, const float *sq, float wx0, float wx1,
+                                               float wy0, float wy1)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int sh = 16; sh >= 0; sh -= 8) {
+        uint32_t X1 = mix_channel_exact(wx0, sq8(sq, (q.x >> sh) & 255u), wx1,
+                                        sq8(sq, (q.y >> sh) & 255u));
+        uint32_t X2 = mix_channel_exact(wx0, sq8(sq, (q.z >> sh) & 255u), wx1,
+                                        sq8(sq, (q.w >> sh) & 255u));
+        out |= mix_channel_exact(wy0, sq8(sq, X1), wy1, sq8(sq, X2)) << sh;
+    }
+    bool opaque_sum = (wx0 + wx1 != 0.0f) && (wy0 + wy1 != 0.0f);
+    return out | (opaque_sum ? 0xFF000000u : 0u);
+}
+
+/* png_color_filtered (fut/render_functions.fut:95-105) evaluated channel
+ * by channel: argb.mix is separable per channel once its weights are
+ * normalised, so the two row mixes feed the column mix as bytes.
+ *
+ * argb.mix divides the weights by their sum m12.  A sum of exactly 1
+ * makes the divisions exact no-ops; a sum of 0 (integer coordinate) makes
+ * the reference's channels 0/0 = NaN, which quantise to byte 0 exactly as
+ * the undivided zero weights do.  Bilinear weights sum to 0 or 1 whenever
+ * |x|, |y| >= 1, so the divisions run only on waves where some lane has
+ * another sum (a coordinate within 1 of the map origin); such waves, and
+ * waves with translucent texels, take the general path below. */
+template <bool CP2>
+__device__ __forceinline__ uint32_t color_bilinear_lut(const fsk_map &m, const float *sq,
+                                                       float x, float y)
+{
+    float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
+    int32_t ix = wrap<CP2>(idx_of(fx), m.ccols);
+    int32_t iy = wrap<CP2>(idx_of(fy), m.crows);
+    uint4 q = ((const uint4 *)m.cq)[(size_t)iy * (size_t)m.ccols + (size_t)ix];
+    float wx0 = cx - x, wx1 = x - fx, wy0 = cy - y, wy1 = y - fy;
+    bool easy = (fabsf(x) >= 1.0f) && (fabsf(y) >= 1.0f) &&
+                (((q.x & q.y & q.z & q.w) >> 24) == 255u);
+    if (__ballot(!easy) == 0)
+        return color_fast(q, sq, wx0, wx1, wy0, wy1);
+    float mx = wx0 + wx1, my = wy0 + wy1;
+    bool odd = !((mx == 1.0f || mx == 0.0f) && (my == 1.0f || my == 0.0f));
+    float nx0 = wx0, nx1 = wx1, ny0 = wy0, ny1 = wy1;
+    if (__ballot(odd)) {
+        if (odd) {
+            nx0 = wx0 / mx;
+            nx1 = wx1 / mx;
+            ny0 = wy0 / my;
+            ny1 = wy1 / my;
+        }
+    }
+    uint32_t out = 0;
+#pragma unroll
+    for (int sh = 16; sh >= 0; sh -= 8) {
+        uint32_t X1 = mix_channel(nx0, sq8(sq, (q.x >> sh) & 255u), nx1, sq8(sq, (q.y >> sh) & 255u));
+        uint32_t X2 = mix_channel(nx0, sq8(sq, (q.z >> sh) & 255u), nx1, sq8(sq, (q.w >> sh) & 255u));
+        out |= mix_channel(ny0, sq8(sq, X1), ny1, sq8(sq, X2)) << sh;
+    }
+    /* alpha: (m1*a1 + m2*a2) / m12, mixed linearly */
+    float ax1 = (wx0 * unit8(q.x >> 24)) + (wx1 * unit8(q.y >> 24));
+    float ax2 = (wx0 * unit8(q.z >> 24)) + (wx1 * unit8(q.w >> 24));
+    if (__ballot(odd)) {
+        if (odd) {
+            ax1 = ax1 / mx;
+            ax2 = ax2 / mx;
+        }
+    }
+    float ay = (wy0 * unit8(chan_u8(ax1))) + (wy1 * unit8(chan_u8(ax2)));
+    if (__ballot(odd)) {
+        if (odd)
+            ay = ay / my;
+    }
+    return out | (chan_u8(ay) << 24);
+}
+
+constexpr int kShortSpan = 24;   /* span rows a lane paints itself */
+
+template <int TC>
+size_t render_lds_bytes_tc(int32_t rows)
+{
+    size_t hp = (size_t)((rows + 3) & ~3);
+    size_t q = (size_t)TC * (size_t)(rows < kChunk ? rows : kChunk);
+    size_t longs = (size_t)TC * (size_t)(rows / kShortSpan + 1);
+    return sizeof(float) * 5 * kTabStride +
+           sizeof(uint32_t) * (hp * TC + q + (q + 1) / 2 + 2 * longs);
+}
+
+/* One workgroup renders a stripe of TC adjacent columns over all slices,
+ * chunk by chunk:
+ *   0. the chunk's slice table (position, step, inv_z) into LDS;
+ *   1. lane (slice group g, column c) projects its kr consecutive slices;
+ *   2. prefix-min over slice groups (wave scan, then a scan of the wave
+ *      totals) gives the y-buffer in front of each lane's slices;
+ *   3. samples strictly above it are visible (scan occlude + scatter,
+ *      fut/voxel_renderer.fut:231, :244): they are queued and their rows
+ *      marked in a per-column bitmask;
+ *   4. each queued sample gets its bilinear argb colour and paints its span
+ *      down to the next visible row below (the reference's top-down
+ *      `scan fill_vline`, :246-249, seen from the sample's side).
+ * Finally the stripe is written row by row with 0 replaced by the sky.  A
+ * sample whose colour is 0 is transparent for fill_vline; if one occurs the
+ * stripe is instead filled with a top-down scan over all rows. */
+template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+__global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
+{
+    constexpr int SG = kThreads / TC;            /* slice groups */
+    constexpr int KR = kChunk / SG;              /* slices per lane per chunk, at most */
     extern __shared__ uint32_t lds[];
-    uint32_t *stripe = lds;                                  /* [rows][kTC] */
-    uint32_t *queue = lds + (size_t)a.rows * kTC;            /* [rows*kTC]  */
-    __shared__ int32_t wtot[2][kWaves][kTC];
-    __shared__ uint32_t wfill[kWaves][kTC];
-    __shared__ uint32_t qn;
+    float4 *tl = (float4 *)lds;                                  /* [kTabStride] */
+    float *tz = (float *)(lds + 4 * kTabStride);                 /* [kTabStride] */
+    const int32_t H = a.f.rows, n = a.f.n;
+    const int32_t Hp = (H + 3) & ~3;
+    uint32_t *stripe = lds + 5 * kTabStride;                     /* [TC][Hp] */
+    const int32_t qcap = TC * min(H, kChunk);                    /* visible rows differ */
+    uint32_t *queue = stripe + (size_t)Hp * TC;                  /* [qcap] */
+    uint16_t *qbot = (uint16_t *)(queue + qcap);                 /* [qcap] span bottoms */
+    uint32_t *longs = queue + qcap + (qcap + 1) / 2;             /* [2][TC*(H/24+1)] */
+    __shared__ float sq[256];
+    __shared__ int32_t wtot[kWaves][TC];
+    __shared__ int32_t wpre[kWaves][TC];
+    __shared__ int32_t carry[TC];
+    __shared__ uint32_t qn, nlong, transparent;
 
     const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t tile = xcd_tile(blockIdx.x, a.ntiles);
-    const int32_t col0 = tile * kTC;
-    const int32_t c = tid % kTC, g = tid / kTC;
+    const int32_t col0 = tile * TC;
+    const int32_t c = tid % TC, g = tid / TC;
     const int32_t col = col0 + c;
-    const bool colok = col < a.cols;
+    const bool colok = col < a.f.cols;
     const float fcol = (float)col;
-    const int32_t H = a.rows;
+    const float Hf = (float)H;
 
-    for (int32_t i = tid; i < H * kTC; i += kThreads)
+    for (int32_t i = tid; i < Hp * TC; i += kThreads)
         stripe[i] = 0u;
-    if (tid == 0)
-        qn = 0u;
-    __syncthreads();
+    if (tid < TC)
+        carry[tid] = H;
+    if (tid == 0) {
+        nlong = 0u;
+        transparent = 0u;
+    }
+    fill_square_lut(sq, tid);
 
-    int32_t carry[kTC];
-#pragma unroll
-    for (int j = 0; j < kTC; j++)
-        carry[j] = H;
+    const int32_t nch = (n + kChunk - 1) / kChunk;
+    const int32_t kr = nch > 0 ? (n + nch * SG - 1) / (nch * SG) : 0;
+    const int32_t csz = kr * SG;
 
-    int32_t it = 0;
-    for (int32_t base = 0; base < a.n; base += kChunk, it ^= 1) {
-        /* phase 1: screen rows of this lane's KR slices */
-        int32_t ah[kKR];
+    for (int32_t base = 0; base < n; base += csz) {
+        const int32_t cn = min(csz, n - base);
+        /* phase 0 */
+        for (int32_t s = tid; s < cn; s += kThreads) {
+            Slice sl = slice_params(a.f, base + s);
+            int32_t o = skew(s);
+            tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
+            tz[o] = sl.iz;
+        }
+        if (tid == 0)
+            qn = 0u;
+        __syncthreads();
+
+        /* phase 1: rows are < 2^16, two per register */
+        uint32_t ah[KR / 2];
         int32_t lmin = H;
 #pragma unroll
-        for (int j = 0; j < kKR; j++) {
-            int32_t k = base + g * kKR + j;
+        for (int j = 0; j < KR; j++) {
+            int32_t s = g * kr + j;
             int32_t r = H;
-            if (colok && k < a.n) {
-                float4 L = a.lines[k];
-                float iz = a.invz[k];
+            if (j < kr && s < cn && colok) {
+                int32_t o = skew(s);
+                float4 L = tl[o];
                 float x = L.x + (fcol * L.z);
                 float y = L.y + (fcol * L.w);
                 float hgt = height_bilinear<HBITS, HP2>(a.m, x, y);
-                float rel = ((a.cam_h - hgt) * iz) + a.horizon;
-                r = min(screen_row(rel), H);
+                float rel = ((a.f.cam_h - hgt) * tz[o]) + a.f.horizon;
+                r = screen_row_clamped(rel, Hf);
             }
-            ah[j] = r;
+            if (j & 1)
+                ah[j / 2] |= (uint32_t)r << 16;
+            else
+                ah[j / 2] = (uint32_t)r;
             lmin = min(lmin, r);
         }
-        /* phase 2: exclusive prefix-min over slice groups of my column */
+
+        /* phase 2 */
         int32_t v = lmin;
 #pragma unroll
-        for (int off = kTC; off < 64; off <<= 1) {
+        for (int off = TC; off < 64; off <<= 1) {
             int32_t o = __shfl_up(v, off);
             if (lane >= off)
                 v = min(v, o);
         }
-        int32_t excl = __shfl_up(v, kTC);
-        if (lane < kTC)
+        int32_t excl = __shfl_up(v, TC);
+        if (lane < TC)
             excl = H;
-        if (lane >= 64 - kTC)
-            wtot[it][wave][c] = v;
+        if (lane >= 64 - TC)
+            wtot[wave][c] = v;
         __syncthreads();
-        int32_t run = carry[c];
-        for (int w2 = 0; w2 < wave; w2++)
-            run = min(run, wtot[it][w2][c]);
-        run = min(run, excl);
-        /* phase 3: samples strictly above the y-buffer are visible */
+        if (wave == 0) {
+            /* lane = w*4 + j handles column j (and j+4 when TC == 8) of
+             * wave w: exclusive prefix over the 16 waves */
 #pragma unroll
-        for (int j = 0; j < kKR; j++) {
-            bool vis = ah[j] < run;
-            if (vis)
-                run = ah[j];
-            unsigned long long mask = __ballot(vis);
-            if (mask) {
-                uint32_t cnt = (uint32_t)__popcll(mask);
-                uint32_t b0 = 0;
-                if (lane == 0)
-                    b0 = atomicAdd(&qn, cnt);
-                b0 = __shfl(b0, 0);
-                if (vis) {
-                    uint32_t off = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                    int32_t k = base + g * kKR + j;
-                    queue[b0 + off] = ((uint32_t)c << 16) | (uint32_t)ah[j];
-                    stripe[ah[j] * kTC + c] = (uint32_t)k + 1u;
+            for (int half = 0; half < TC / 4; half++) {
+                int32_t cc = (lane & 3) + 4 * half;
+                int32_t t = wtot[lane >> 2][cc];
+#pragma unroll
+                for (int off = 4; off < 64; off <<= 1) {
+                    int32_t o = __shfl_up(t, off);
+                    if (lane >= off)
+                        t = min(t, o);
+                }
+                int32_t ex = __shfl_up(t, 4);
+                wpre[lane >> 2][cc] = min(carry[cc], lane < 4 ? H : ex);
+            }
+        }
+        __syncthreads();
+        int32_t run = min(wpre[wave][c], excl);
+
+        /* phase 3: mark this lane's visible samples, reserve queue slots
+         * with one wave-level prefix sum, then queue each sample with the
+         * bottom of its span -- the y-buffer just before it, i.e. the next
+         * visible row below in its column */
+        const int32_t run_in = run;
+        uint32_t vbits = 0u;
+#pragma unroll
+        for (int j = 0; j < KR; j++) {
+            int32_t aj = (int32_t)((ah[j / 2] >> ((j & 1) * 16)) & 0xFFFFu);
+            bool vis = aj < run;
+            run = vis ? aj : run;
+            vbits |= vis ? (1u << j) : 0u;
+        }
+        {
+            uint32_t cnt = (uint32_t)__builtin_popcount(vbits);
+            uint32_t incl = cnt;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                uint32_t o = __shfl_up(incl, off);
+                if (lane >= off)
+                    incl += o;
+            }
+            uint32_t total = __shfl(incl, 63);
+            uint32_t b0 = 0;
+            if (lane == 0 && total)
+                b0 = atomicAdd(&qn, total);
+            b0 = __builtin_amdgcn_readfirstlane(b0);
+            uint32_t slot = b0 + incl - cnt;
+            int32_t prev = run_in;
+#pragma unroll
+            for (int j = 0; j < KR; j++) {
+                if (vbits & (1u << j)) {
+                    int32_t aj = (int32_t)((ah[j / 2] >> ((j & 1) * 16)) & 0xFFFFu);
+                    queue[slot] = ((uint32_t)(g * kr + j) << 16) | ((uint32_t)c << 12) |
+                                  (uint32_t)aj;
+                    qbot[slot] = (uint16_t)prev;
+                    prev = aj;
+                    slot++;
                 }
             }
         }
+        /* the last slice group of each column holds the running minimum of
+         * the whole chunk: the carry into the next chunk */
+        if (g == SG - 1)
+            carry[c] = run;
+        __syncthreads();
         bool done = true;
 #pragma unroll
-        for (int j = 0; j < kTC; j++) {
-            for (int w2 = 0; w2 < kWaves; w2++)
-                carry[j] = min(carry[j], wtot[it][w2][j]);
-            done = done && (carry[j] == 0);
+        for (int j = 0; j < TC; j++)
+            done = done && carry[j] == 0;
+
+        /* phase 4: colour and span */
+        const uint32_t nq = qn;
+        for (uint32_t i = tid; i < nq; i += kThreads) {
+            uint32_t e = queue[i];
+            int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
+            int32_t row = (int32_t)(e & 0xFFFu);
+            float4 L = tl[skew(s)];
+            float fc = (float)(col0 + cc);
+            float x = L.x + (fc * L.z);
+            float y = L.y + (fc * L.w);
+            uint32_t colour = color_bilinear_lut<CP2>(a.m, sq, x, y);
+            if (colour == 0u) {
+                transparent = 1u;
+                continue;
+            }
+            int32_t bottom = (int32_t)qbot[i];
+            uint32_t *dst = stripe + cc * Hp;
+            int32_t stop = min(bottom, row + kShortSpan);
+            for (int32_t r = row; r < stop; r++)
+                dst[r] = colour;
+            if (bottom > stop) {
+                uint32_t li = atomicAdd(&nlong, 1u);
+                longs[2 * li] = ((uint32_t)cc << 26) | ((uint32_t)stop << 13) | (uint32_t)bottom;
+                longs[2 * li + 1] = colour;
+            }
         }
+        __syncthreads();
         if (done)
             break;
     }
-    __syncthreads();
 
-    /* phase 4: colour of every visible sample */
-    const uint32_t nq = qn;
-    for (uint32_t i = tid; i < nq; i += kThreads) {
-        uint32_t e = queue[i];
-        int32_t cc = (int32_t)(e >> 16), row = (int32_t)(e & 0xFFFFu);
-        int32_t k = (int32_t)stripe[row * kTC + cc] - 1;
-        float4 L = a.lines[k];
-        float fc = (float)(col0 + cc);
-        float x = L.x + (fc * L.z);
-        float y = L.y + (fc * L.w);
-        stripe[row * kTC + cc] = color_bilinear<CP2>(a.m, x, y);
+    /* long spans, painted by the whole workgroup */
+    const uint32_t nl = nlong;
+    for (uint32_t li = 0; li < nl; li++) {
+        uint32_t d = longs[2 * li], colour = longs[2 * li + 1];
+        int32_t cc = (int32_t)(d >> 26), r0 = (int32_t)((d >> 13) & 0x1FFFu);
+        int32_t r1 = (int32_t)(d & 0x1FFFu);
+        for (int32_t r = r0 + tid; r < r1; r += kThreads)
+            stripe[cc * Hp + r] = colour;
     }
-    __syncthreads();
+    if (nl)
+        __syncthreads();
 
-    /* phase 5: top-down fill (last non-zero at or above each row), sky,
-     * and the row-major write-out */
+    const bool vec = (col0 + TC <= a.f.cols) && ((a.pitch & 3) == 0) &&
+                     ((((uintptr_t)a.out) & 15u) == 0);
+    if (transparent == 0u) {
+        /* phase 5: every non-zero cell is final; 0 is sky.  TC/4 adjacent
+         * lanes share a row, so each store instruction writes whole
+         * TC*4-byte row segments */
+        constexpr int PARTS = TC / 4;
+        const int32_t part = tid % PARTS;
+        for (int32_t r = tid / PARTS; r < H; r += kThreads / PARTS) {
+            uint32_t px[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t s = stripe[(part * 4 + j) * Hp + r];
+                px[j] = s ? s : a.f.sky;
+            }
+            uint32_t *dst = a.out + (size_t)r * (size_t)a.pitch + col0 + part * 4;
+            if (vec) {
+                *(uint4 *)dst = make_uint4(px[0], px[1], px[2], px[3]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (col0 + part * 4 + j < a.f.cols)
+                        dst[j] = px[j];
+            }
+        }
+        return;
+    }
+
+    /* phase 5 with transparent samples: a zero cell takes the last
+     * non-zero colour above it (fill_vline), else the sky; each thread
+     * owns a block of consecutive rows */
     const int32_t per = (H + kThreads - 1) / kThreads;
     const int32_t r0 = min(tid * per, H), r1 = min(r0 + per, H);
-    uint32_t last[kTC];
+    uint32_t pre[TC];
 #pragma unroll
-    for (int j = 0; j < kTC; j++)
-        last[j] = 0u;
-    for (int32_t r = r0; r < r1; r++) {
-#pragma unroll
-        for (int j = 0; j < kTC; j++) {
-            uint32_t s = stripe[r * kTC + j];
-            last[j] = s ? s : last[j];
+    for (int j = 0; j < TC; j++) {
+        uint32_t p = 0u;
+        for (int32_t r = r0; r < r1; r++) {
+            uint32_t s = stripe[j * Hp + r];
+            p = s ? s : p;
         }
+        pre[j] = p;
     }
-    uint32_t pre[kTC];
+    uint32_t *wfill = (uint32_t *)&wtot[0][0];      /* [kWaves][TC] */
+    uint32_t *wfpre = (uint32_t *)&wpre[0][0];
 #pragma unroll
-    for (int j = 0; j < kTC; j++) {
-        uint32_t v = last[j];
+    for (int j = 0; j < TC; j++) {
+        uint32_t v = pre[j];
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             uint32_t o = __shfl_up(v, off);
@@ -360,37 +688,49 @@ __global__ __launch_bounds__(kThreads) void k_render(RenderArgs a)
                 v = v ? v : o;
         }
         uint32_t ex = __shfl_up(v, 1);
-        if (lane == 0)
-            ex = 0u;
+        pre[j] = lane == 0 ? 0u : ex;
         if (lane == 63)
-            wfill[wave][j] = v;
-        pre[j] = ex;
+            wfill[wave * TC + j] = v;
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int half = 0; half < TC / 4; half++) {
+            int32_t cc = (lane & 3) + 4 * half;
+            uint32_t t = wfill[(lane >> 2) * TC + cc];
+#pragma unroll
+            for (int off = 4; off < 64; off <<= 1) {
+                uint32_t o = __shfl_up(t, off);
+                if (lane >= off)
+                    t = t ? t : o;
+            }
+            uint32_t ex = __shfl_up(t, 4);
+            wfpre[(lane >> 2) * TC + cc] = lane < 4 ? 0u : ex;
+        }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kTC; j++) {
-        uint32_t p = 0u;
-        for (int w2 = 0; w2 < wave; w2++)
-            p = wfill[w2][j] ? wfill[w2][j] : p;
+    for (int j = 0; j < TC; j++) {
+        uint32_t p = wfpre[wave * TC + j];
         pre[j] = pre[j] ? pre[j] : p;
     }
-    const bool vec = (col0 + kTC <= a.cols) && ((a.pitch & 3) == 0) &&
-                     ((((uintptr_t)a.out) & 15u) == 0);
     for (int32_t r = r0; r < r1; r++) {
-        uint32_t px[kTC];
+        uint32_t px[TC];
 #pragma unroll
-        for (int j = 0; j < kTC; j++) {
-            uint32_t s = stripe[r * kTC + j];
+        for (int j = 0; j < TC; j++) {
+            uint32_t s = stripe[j * Hp + r];
             pre[j] = s ? s : pre[j];
-            px[j] = pre[j] ? pre[j] : a.sky;
+            px[j] = pre[j] ? pre[j] : a.f.sky;
         }
         uint32_t *dst = a.out + (size_t)r * (size_t)a.pitch + col0;
         if (vec) {
-            *(uint4 *)dst = make_uint4(px[0], px[1], px[2], px[3]);
+#pragma unroll
+            for (int j = 0; j < TC; j += 4)
+                *(uint4 *)(dst + j) = make_uint4(px[j], px[j + 1], px[j + 2], px[j + 3]);
         } else {
 #pragma unroll
-            for (int j = 0; j < kTC; j++)
-                if (col0 + j < a.cols)
+            for (int j = 0; j < TC; j++)
+                if (col0 + j < a.f.cols)
                     dst[j] = px[j];
         }
     }
@@ -468,21 +808,131 @@ __global__ void k_shade(const uint16_t *__restrict__ hp, int32_t hrows, int32_t 
     out[(size_t)y * size + x] = argb_mix(step_size * amount, 0xFF000000u, 1.0f, c);
 }
 
-template <int HBITS, bool HP2, bool CP2>
-hipError_t launch_render(const RenderArgs &a, size_t lds, hipStream_t s)
+/* ------------------------------------------------------------------ */
+/* Device self-tests of the arithmetic shortcuts                       */
+/* ------------------------------------------------------------------ */
+
+/* 0: unit8(b) == b / 255 (correctly rounded) for all bytes.
+ * 1: sqrt_unit(t) == correctly rounded sqrt(t) for every f32 in [0, 1]
+ *    and for NaNs.
+ * 2: chan_u8(v) == u32.f32(clamp_channel(v) * 255) with the if-chain
+ *    clamp, for every f32 bit pattern.
+ * 3: screen_row_clamped(rel, H) == min(screen_row(rel), H) for every f32
+ *    bit pattern at H = 1080. */
+__global__ void k_selftest(int which, unsigned long long *bad)
 {
-    auto kern = k_render<HBITS, HP2, CP2>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(2 * kMaxRows * kTC * sizeof(uint32_t)));
+    uint64_t n = which == 0 ? 256ull : which == 1 ? 0x3F800001ull : 0x100000000ull;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long local = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t u = (uint32_t)i;
+        bool ok = true;
+        if (which == 0) {
+            ok = __float_as_uint(unit8(u)) == __float_as_uint((float)u / 255.0f);
+        } else if (which == 1) {
+            float t = __uint_as_float(u);
+            ok = __float_as_uint(sqrt_unit(t)) == __float_as_uint(__builtin_sqrtf(t));
+            float nan = __uint_as_float(0x7FC00000u | (u & 0x3FFFFFu));
+            ok = ok && (sqrt_unit(nan) != sqrt_unit(nan));
+        } else if (which == 2) {
+            float v = __uint_as_float(u);
+            float c = (v < 0.0f) ? 0.0f : ((v > 1.0f) ? 1.0f : v);
+            float sc = c * 255.0f;
+            uint32_t want = (sc == sc) ? (uint32_t)sc : 0u;
+            ok = chan_u8(v) == want;
+        } else {
+            float rel = __uint_as_float(u);
+            ok = screen_row_clamped(rel, 1080.0f) == min(screen_row(rel), 1080);
+        }
+        if (!ok) {
+            local += 1;
+            atomicMin(bad + 1, (unsigned long long)u);
+        }
+    }
+    if (local)
+        atomicAdd(bad, local);
+}
+
+template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
+{
+    auto kern = k_render<TC, HBITS, HP2, CP2, WPE>;
+    static int dyn_max = -1;
+    if (dyn_max < 0) {
+        /* the whole LDS of a CU minus the kernel's static shared memory */
+        int dev = 0, lds = 0;
+        hipFuncAttributes fa;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess)
+            e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+        if (e == hipSuccess)
+            e = hipFuncGetAttributes(&fa, (const void *)kern);
         if (e != hipSuccess)
             return e;
-        attr_set = true;
+        int want = lds - (int)fa.sharedSizeBytes;
+        e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+        if (e != hipSuccess)
+            return e;
+        dyn_max = want;
     }
-    hipLaunchKernelGGL(kern, dim3(a.ntiles), dim3(kThreads), lds, s, a);
+    size_t need = render_lds_bytes_tc<TC>(a.f.rows);
+    if (need > (size_t)dyn_max)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kern, dim3(a.ntiles), dim3(kThreads), need, s, a);
     return hipGetLastError();
+}
+
+/* Render kernel shape: columns per workgroup (8: 32-byte row segments,
+ * one 1024-thread workgroup per CU; 4: 16-byte segments, for frames whose
+ * 8-column stripe does not fit in LDS) and, for 4 columns, the occupancy
+ * the registers are budgeted for (1 or 2 workgroups per CU).
+ * FSX_RENDER_TC (4 or 8) and FSX_RENDER_WG_PER_CU (1 or 2) override. */
+struct RenderShape {
+    int tc, wg_per_cu;
+};
+
+inline size_t lds_per_cu()
+{
+    static size_t v = 0;
+    if (v == 0) {
+        int dev = 0, lds = 65536;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+                hipSuccess)
+            lds = 65536;
+        v = (size_t)lds;
+    }
+    return v;
+}
+
+inline RenderShape render_shape(int32_t rows)
+{
+    static int env_tc = -1, env_wg = -1;
+    if (env_tc < 0) {
+        const char *e = getenv("FSX_RENDER_TC");
+        env_tc = (e && e[0] == '8') ? 8 : (e && e[0] == '4') ? 4 : 0;
+        e = getenv("FSX_RENDER_WG_PER_CU");
+        env_wg = (e && e[0] == '1') ? 1 : 2;
+    }
+    RenderShape rs;
+    /* 4 KB of headroom for the kernel's static shared memory */
+    bool fits8 = render_lds_bytes_tc<8>(rows) + 4096 <= lds_per_cu();
+    rs.tc = env_tc == 4 ? 4 : (env_tc == 8 || fits8) && fits8 ? 8 : 4;
+    rs.wg_per_cu = rs.tc == 8 ? 1 : env_wg;
+    return rs;
+}
+
+template <int HBITS, bool HP2, bool CP2>
+hipError_t launch_render(RenderArgs a, hipStream_t s)
+{
+    RenderShape rs = render_shape(a.f.rows);
+    if (rs.tc == 8) {
+        a.ntiles = (a.f.cols + 7) / 8;
+        return launch_render_cfg<8, HBITS, HP2, CP2, 4>(a, s);
+    }
+    a.ntiles = (a.f.cols + 3) / 4;
+    return rs.wg_per_cu == 1 ? launch_render_cfg<4, HBITS, HP2, CP2, 4>(a, s)
+                             : launch_render_cfg<4, HBITS, HP2, CP2, 8>(a, s);
 }
 
 inline bool pow2(int32_t v)
@@ -496,12 +946,20 @@ extern "C" {
 
 size_t fsk_render_lds_bytes(int32_t rows)
 {
-    return (size_t)2 * (size_t)rows * kTC * sizeof(uint32_t);
+    return render_shape(rows).tc == 8 ? render_lds_bytes_tc<8>(rows) : render_lds_bytes_tc<4>(rows);
 }
 
 int32_t fsk_render_max_rows(void)
 {
     return kMaxRows;
+}
+
+int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s)
+{
+    if (which < 0 || which > 3)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_selftest, dim3(4096), dim3(256), 0, s, which, d_bad);
+    return hipGetLastError();
 }
 
 int fsk_slices(const fsk_frame *f, float *d_tab, hipStream_t s)
@@ -514,8 +972,8 @@ int fsk_slices(const fsk_frame *f, float *d_tab, hipStream_t s)
     return hipGetLastError();
 }
 
-int fsk_render(const fsk_map *m, const fsk_frame *f, const float *d_tab,
-               uint32_t *d_out, int64_t pitch, hipStream_t s)
+int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
+               hipStream_t s)
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
@@ -523,31 +981,25 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, const float *d_tab,
         return hipErrorInvalidValue;
     RenderArgs a;
     a.m = *m;
-    a.lines = (const float4 *)d_tab;
-    a.invz = d_tab + 4 * (size_t)(f->n > 0 ? f->n : 0);
+    a.f = *f;
+    if (a.f.n < 0)
+        a.f.n = 0;
     a.out = d_out;
     a.pitch = pitch;
-    a.n = f->n > 0 ? f->n : 0;
-    a.rows = f->rows;
-    a.cols = f->cols;
-    a.ntiles = (f->cols + kTC - 1) / kTC;
-    a.cam_h = f->cam_h;
-    a.horizon = f->horizon;
-    a.sky = f->sky;
-    size_t lds = fsk_render_lds_bytes(f->rows);
+    a.ntiles = 0;
     bool hp2 = pow2(m->hrows) && pow2(m->hcols);
     bool cp2 = pow2(m->crows) && pow2(m->ccols);
     hipError_t e;
     if (m->hbits == 8) {
-        if (hp2 && cp2) e = launch_render<8, true, true>(a, lds, s);
-        else if (hp2) e = launch_render<8, true, false>(a, lds, s);
-        else if (cp2) e = launch_render<8, false, true>(a, lds, s);
-        else e = launch_render<8, false, false>(a, lds, s);
+        if (hp2 && cp2) e = launch_render<8, true, true>(a, s);
+        else if (hp2) e = launch_render<8, true, false>(a, s);
+        else if (cp2) e = launch_render<8, false, true>(a, s);
+        else e = launch_render<8, false, false>(a, s);
     } else {
-        if (hp2 && cp2) e = launch_render<16, true, true>(a, lds, s);
-        else if (hp2) e = launch_render<16, true, false>(a, lds, s);
-        else if (cp2) e = launch_render<16, false, true>(a, lds, s);
-        else e = launch_render<16, false, false>(a, lds, s);
+        if (hp2 && cp2) e = launch_render<16, true, true>(a, s);
+        else if (hp2) e = launch_render<16, true, false>(a, s);
+        else if (cp2) e = launch_render<16, false, true>(a, s);
+        else e = launch_render<16, false, false>(a, s);
     }
     return e;
 }

@@ -183,6 +183,13 @@ int fsx_profile_read(struct futhark_context *ctx, double *total_ms, int64_t *lau
  * sun_height (fut/interactive.fut:163). */
 uint32_t fsx_sky(float sun_height);
 
+/* Run device self-test `which` of the kernels' arithmetic shortcuts
+ * (0: byte/255, 1: sqrt on [0,1], 2: channel quantisation, 3: screen-row
+ * clamp) and report how many inputs disagree with the plain formula and
+ * the smallest such input's bit pattern (-1 when none; may be NULL). */
+int fsx_selftest(struct futhark_context *ctx, int which, int64_t *mismatches,
+                 int64_t *first_bad);
+
 /* Version string of this library. */
 const char *fsx_version(void);
 

@@ -172,3 +172,15 @@ def test_batch_matches_single_views(ctx):
     for v, o in zip(views, offs):
         got = flat[o:o + v.h * v.w].reshape(v.h, v.w)
         _check(got, ctx.render(maps[v.map], v))
+
+
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
+def test_device_arithmetic_selftest(ctx, which):
+    """The kernels' shortcuts (byte/255 by reciprocal + FMA correction, sqrt
+    on [0,1] without scaling, median-based quantisation and row clamp) agree
+    with the plain formulas on every input they can see."""
+    import ctypes
+    bad, first = ctypes.c_int64(-1), ctypes.c_int64(-1)
+    ctx.check(ctx.lib.fsx_selftest(ctx.ptr, which, ctypes.byref(bad), ctypes.byref(first)),
+              "fsx_selftest")
+    assert bad.value == 0, f"{bad.value} mismatches, first input bits {first.value:#x}"
