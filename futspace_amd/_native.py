@@ -101,6 +101,13 @@ FUTSPACE_SIGNATURES = {
     "fsx_render": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, I64]),
     "fsx_render_host": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P]),
     "fsx_render_batch": (ctypes.c_int, [P, P, I32, ctypes.POINTER(FsxView), I32, P, P]),
+    "fsx_batch_new": (ctypes.c_int, [P, P, I32, ctypes.POINTER(FsxView), I32, P, P,
+                                     ctypes.POINTER(P)]),
+    "fsx_batch_run": (ctypes.c_int, [P, P]),
+    "fsx_batch_capture": (ctypes.c_int, [P, P]),
+    "fsx_batch_replay": (ctypes.c_int, [P, P]),
+    "fsx_batch_free": (ctypes.c_int, [P, P]),
+    "fsx_set_tuning": (ctypes.c_int, [P, I32, I32, I32]),
     "fsx_context_set_stream": (ctypes.c_int, [P, P]),
     "fsx_context_device": (ctypes.c_int, [P]),
     "fsx_version": (ctypes.c_char_p, []),

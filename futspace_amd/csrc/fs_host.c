@@ -53,6 +53,7 @@ struct futhark_context {
     } pool[FS_POOL_SLOTS];    /* freed output buffers kept for reuse */
     struct fs_terrain *empty_terrain;
     struct fs_shade *empty_shade;
+    fsk_tuning tune;          /* render kernel shape (0 = automatic) */
     int prof_on;              /* bracket render kernels with events */
     int prof_n;
     hipEvent_t *prof_ev;      /* FS_PROF_MAX pairs */
@@ -1346,20 +1347,299 @@ int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,
     return rc;
 }
 
+/* ------------------------------------------------------------------ */
+/* Batched renders                                                     */
+/* ------------------------------------------------------------------ */
+
+#define FS_QUEUES 8   /* one work queue per XCD */
+
+struct fsx_batch {
+    fsk_batch b;
+    void *d_mem;                   /* views | items | qmeta | counters */
+    int32_t n_views, n_maps;
+    struct fs_terrain **terrains;  /* references held while the batch lives */
+    struct fs_shade **shades;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int fsx_set_tuning(struct futhark_context *ctx, int32_t tile_columns, int32_t wg_per_cu,
+                   int32_t chunk)
+{
+    if (!ctx)
+        return 1;
+    if ((tile_columns != 0 && tile_columns != 4 && tile_columns != 8) ||
+        (wg_per_cu != 0 && wg_per_cu != 1 && wg_per_cu != 2) ||
+        (chunk != 0 && (chunk < 256 || chunk > 2048))) {
+        set_error(ctx, "fsx_set_tuning: unsupported tuning (%d, %d, %d)", tile_columns, wg_per_cu,
+                  chunk);
+        return 1;
+    }
+    fsk_tuning t = {tile_columns, wg_per_cu, chunk};
+    ctx->tune = t;
+    fsk_set_tuning(&t);
+    return 0;
+}
+
+static int is_pow2(int32_t v)
+{
+    return v > 0 && (v & (v - 1)) == 0;
+}
+
+/* Deal the views' tiles to the per-XCD queues: costlier views first, each
+ * view's tiles cut into FS_QUEUES contiguous blocks so that neighbouring
+ * tiles share an XCD. */
+static void deal_items(const fsk_batch_view *bv, int32_t n_views, uint32_t *items,
+                       uint32_t *qmeta)
+{
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_views > 0 ? n_views : 1));
+    for (int32_t i = 0; i < n_views; i++)
+        order[i] = i;
+    /* insertion sort by descending work (slices x columns + pixels) */
+    for (int32_t i = 1; i < n_views; i++) {
+        int32_t k = order[i];
+        double wk = (double)bv[k].f.n * bv[k].f.cols + (double)bv[k].f.rows * bv[k].f.cols;
+        int32_t j = i - 1;
+        while (j >= 0) {
+            int32_t o = order[j];
+            double wo = (double)bv[o].f.n * bv[o].f.cols + (double)bv[o].f.rows * bv[o].f.cols;
+            if (wo >= wk)
+                break;
+            order[j + 1] = o;
+            j--;
+        }
+        order[j + 1] = k;
+    }
+    uint32_t len[FS_QUEUES] = {0};
+    for (int32_t i = 0; i < n_views; i++) {
+        int32_t nt = bv[order[i]].ntiles;
+        for (int q = 0; q < FS_QUEUES; q++)
+            len[q] += (uint32_t)((int64_t)nt * (q + 1) / FS_QUEUES - (int64_t)nt * q / FS_QUEUES);
+    }
+    uint32_t start = 0, fill[FS_QUEUES];
+    for (int q = 0; q < FS_QUEUES; q++) {
+        qmeta[q] = start;
+        qmeta[FS_QUEUES + q] = len[q];
+        fill[q] = start;
+        start += len[q];
+    }
+    for (int32_t i = 0; i < n_views; i++) {
+        int32_t v = order[i], nt = bv[v].ntiles;
+        for (int q = 0; q < FS_QUEUES; q++) {
+            int32_t t0 = (int32_t)((int64_t)nt * q / FS_QUEUES);
+            int32_t t1 = (int32_t)((int64_t)nt * (q + 1) / FS_QUEUES);
+            for (int32_t t = t0; t < t1; t++)
+                items[fill[q]++] = ((uint32_t)v << 16) | (uint32_t)t;
+        }
+    }
+    free(order);
+}
+
+int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps, int32_t n_maps,
+                  const struct fsx_view *views, int32_t n_views, uint32_t *dev_out,
+                  const int64_t *offsets, struct fsx_batch **out)
+{
+    if (!ctx || !maps || !views || !dev_out || !offsets || !out || n_views < 1 || n_maps < 1)
+        return 1;
+    if (n_views > 65535) {
+        set_error(ctx, "fsx_batch_new: at most 65535 views per batch");
+        return 1;
+    }
+    int32_t max_rows = 0, hbits = -1, pow2 = 1;
+    for (int32_t i = 0; i < n_views; i++) {
+        const struct fsx_view *v = &views[i];
+        if (v->map < 0 || v->map >= n_maps || !maps[v->map]) {
+            set_error(ctx, "fsx_batch_new: view %d names map %d of %d", i, v->map, n_maps);
+            return 1;
+        }
+        if (v->h <= 0 || v->w <= 0 || v->h > fsk_render_max_rows() || v->w > 65535 * 4) {
+            set_error(ctx, "fsx_batch_new: view %d has unsupported size %d x %d", i, v->h, v->w);
+            return 1;
+        }
+        const struct fsx_map *m = maps[v->map];
+        if (hbits < 0)
+            hbits = m->t->hbits;
+        if (m->t->hbits != hbits) {
+            set_error(ctx, "fsx_batch_new: maps with 8- and 16-bit heights in one batch");
+            return 1;
+        }
+        pow2 &= is_pow2(m->t->rows) && is_pow2(m->t->cols) && is_pow2(m->s->rows) &&
+                is_pow2(m->s->cols);
+        if (v->h > max_rows)
+            max_rows = v->h;
+    }
+    struct fsx_batch *bt = (struct fsx_batch *)calloc(1, sizeof *bt);
+    fsk_batch_view *hv = (fsk_batch_view *)calloc((size_t)n_views, sizeof *hv);
+    if (!bt || !hv) {
+        free(bt);
+        free(hv);
+        set_error(ctx, "fsx_batch_new: out of host memory");
+        return 1;
+    }
+    bt->b.tune = ctx->tune;
+    int32_t tc = fsk_tile_columns(max_rows, &bt->b.tune);
+    int64_t n_items = 0;
+    for (int32_t i = 0; i < n_views; i++) {
+        const struct fsx_map *m = maps[views[i].map];
+        hv[i].m.hq = m->t->d_hq;
+        hv[i].m.hrows = m->t->rows;
+        hv[i].m.hcols = m->t->cols;
+        hv[i].m.hbits = m->t->hbits;
+        hv[i].m.cq = m->s->d_cq;
+        hv[i].m.crows = m->s->rows;
+        hv[i].m.ccols = m->s->cols;
+        frame_of_view(&views[i], &hv[i].f);
+        hv[i].offset = offsets[i];
+        hv[i].ntiles = (views[i].w + tc - 1) / tc;
+        n_items += hv[i].ntiles;
+    }
+    size_t vb = sizeof(fsk_batch_view) * (size_t)n_views;
+    size_t ib = sizeof(uint32_t) * (size_t)n_items;
+    size_t mb = sizeof(uint32_t) * 2 * FS_QUEUES;
+    size_t cb = sizeof(uint32_t) * FS_QUEUES;
+    uint32_t *hitems = (uint32_t *)malloc(ib + mb);
+    int rc = 1;
+    if (hitems && hipSetDevice(ctx->device) == hipSuccess &&
+        hipMalloc(&bt->d_mem, vb + ib + mb + cb) == hipSuccess) {
+        deal_items(hv, n_views, hitems, hitems + n_items);
+        char *base = (char *)bt->d_mem;
+        hipError_t e = hipMemcpy(base, hv, vb, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(base + vb, hitems, ib + mb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            bt->b.views = (const fsk_batch_view *)base;
+            bt->b.items = (const uint32_t *)(base + vb);
+            bt->b.qmeta = (const uint32_t *)(base + vb + ib);
+            bt->b.counters = (uint32_t *)(base + vb + ib + mb);
+            bt->b.out = dev_out;
+            bt->b.nqueues = FS_QUEUES;
+            bt->b.max_rows = max_rows;
+            bt->b.hbits = hbits;
+            bt->b.pow2 = pow2;
+            bt->b.grid = fsk_batch_grid(max_rows, &bt->b.tune);
+            rc = 0;
+        } else {
+            set_error(ctx, "fsx_batch_new: upload failed: %s", hipGetErrorString(e));
+        }
+    } else {
+        set_error(ctx, "fsx_batch_new: allocation failed");
+    }
+    free(hitems);
+    free(hv);
+    if (rc == 0) {
+        bt->n_views = n_views;
+        bt->n_maps = n_maps;
+        bt->terrains = (struct fs_terrain **)calloc((size_t)n_maps, sizeof *bt->terrains);
+        bt->shades = (struct fs_shade **)calloc((size_t)n_maps, sizeof *bt->shades);
+        if (!bt->terrains || !bt->shades)
+            rc = 1;
+        for (int32_t i = 0; rc == 0 && i < n_maps; i++) {
+            if (!maps[i])
+                continue;
+            bt->terrains[i] = maps[i]->t;
+            bt->shades[i] = maps[i]->s;
+            maps[i]->t->refs++;
+            maps[i]->s->refs++;
+        }
+    }
+    if (rc) {
+        if (bt->d_mem)
+            hipFree(bt->d_mem);
+        free(bt->terrains);
+        free(bt->shades);
+        free(bt);
+        return 1;
+    }
+    *out = bt;
+    return 0;
+}
+
+int fsx_batch_run(struct futhark_context *ctx, struct fsx_batch *batch)
+{
+    if (!ctx || !batch)
+        return 1;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int pe = prof_begin(ctx);
+    HIPCHECK(ctx, fsk_batch_run(&batch->b, ctx->stream));
+    prof_end(ctx, pe);
+    return 0;
+}
+
+int fsx_batch_capture(struct futhark_context *ctx, struct fsx_batch *batch)
+{
+    if (!ctx || !batch)
+        return 1;
+    if (ctx->stream == NULL) {
+        set_error(ctx, "fsx_batch_capture: the context stream cannot be captured");
+        return 1;
+    }
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    if (batch->exec) {
+        hipGraphExecDestroy(batch->exec);
+        batch->exec = NULL;
+    }
+    if (batch->graph) {
+        hipGraphDestroy(batch->graph);
+        batch->graph = NULL;
+    }
+    HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = (hipError_t)fsk_batch_run(&batch->b, ctx->stream);
+    hipGraph_t g = NULL;
+    hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g)
+            hipGraphDestroy(g);
+        set_error(ctx, "fsx_batch_capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        return 1;
+    }
+    batch->graph = g;
+    HIPCHECK(ctx, hipGraphInstantiate(&batch->exec, g, NULL, NULL, 0));
+    return 0;
+}
+
+int fsx_batch_replay(struct futhark_context *ctx, struct fsx_batch *batch)
+{
+    if (!ctx || !batch || !batch->exec) {
+        if (ctx)
+            set_error(ctx, "fsx_batch_replay: the batch has not been captured");
+        return 1;
+    }
+    int pe = prof_begin(ctx);
+    HIPCHECK(ctx, hipGraphLaunch(batch->exec, ctx->stream));
+    prof_end(ctx, pe);
+    return 0;
+}
+
+int fsx_batch_free(struct futhark_context *ctx, struct fsx_batch *batch)
+{
+    if (!ctx)
+        return 1;
+    if (!batch)
+        return 0;
+    hipStreamSynchronize(ctx->stream);
+    if (batch->exec)
+        hipGraphExecDestroy(batch->exec);
+    if (batch->graph)
+        hipGraphDestroy(batch->graph);
+    hipFree(batch->d_mem);
+    for (int32_t i = 0; i < batch->n_maps; i++) {
+        terrain_unref(ctx, batch->terrains[i]);
+        shade_unref(ctx, batch->shades[i]);
+    }
+    free(batch->terrains);
+    free(batch->shades);
+    free(batch);
+    return 0;
+}
+
 int fsx_render_batch(struct futhark_context *ctx, const struct fsx_map *const *maps,
                      int32_t n_maps, const struct fsx_view *views, int32_t n_views,
                      uint32_t *dev_out, const int64_t *offsets)
 {
-    if (!ctx || !maps || !views || !dev_out || !offsets)
+    struct fsx_batch *b = NULL;
+    if (fsx_batch_new(ctx, maps, n_maps, views, n_views, dev_out, offsets, &b))
         return 1;
-    for (int32_t i = 0; i < n_views; i++) {
-        if (views[i].map < 0 || views[i].map >= n_maps || !maps[views[i].map]) {
-            set_error(ctx, "fsx_render_batch: view %d names map %d of %d", i, views[i].map, n_maps);
-            return 1;
-        }
-        const struct fsx_map *m = maps[views[i].map];
-        if (render_into(ctx, m->t, m->s, &views[i], dev_out + offsets[i], views[i].w))
-            return 1;
-    }
-    return 0;
+    int rc = fsx_batch_run(ctx, b);
+    fsx_batch_free(ctx, b);
+    return rc;
 }

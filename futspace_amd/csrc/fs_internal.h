@@ -48,6 +48,43 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pi
 size_t fsk_render_lds_bytes(int32_t rows);
 int32_t fsk_render_max_rows(void);
 
+/* Tuning knobs of the render kernels (0 = automatic). */
+typedef struct {
+    int32_t tc;          /* frame columns per workgroup tile: 4 or 8 */
+    int32_t wg_per_cu;   /* 1 or 2 workgroups per CU (4-column tiles only) */
+    int32_t chunk;       /* depth slices per chunk: 256..2048 */
+} fsk_tuning;
+
+/* One view of a batched render, as the persistent kernel reads it. */
+typedef struct {
+    fsk_map m;
+    fsk_frame f;
+    int64_t offset;      /* first output element of the view */
+    int32_t ntiles;
+    int32_t pad;
+} fsk_batch_view;
+
+/* A prepared batch: the device arrays stay valid while it is launched. */
+typedef struct {
+    const fsk_batch_view *views;   /* device */
+    const uint32_t *items;         /* device: view << 16 | tile, grouped by queue */
+    const uint32_t *qmeta;         /* device: qstart[nqueues], qlen[nqueues] */
+    uint32_t *counters;            /* device: nqueues claim counters */
+    uint32_t *out;
+    int32_t nqueues, max_rows, hbits, pow2;
+    int32_t grid;                  /* persistent workgroups */
+    fsk_tuning tune;
+} fsk_batch;
+
+/* Columns per tile the kernels use for frames of max_rows rows. */
+int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t);
+/* Persistent workgroups for a batch (CUs x workgroups per CU). */
+int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t);
+/* Reset the claim counters and run the persistent batched render. */
+int fsk_batch_run(const fsk_batch *b, hipStream_t s);
+/* Select the plain render's tuning (process-wide). */
+void fsk_set_tuning(const fsk_tuning *t);
+
 /* Device self-test `which` (0-3) of the arithmetic shortcuts; adds the
  * number of mismatching inputs to *d_bad. */
 int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s);

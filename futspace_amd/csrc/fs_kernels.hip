@@ -271,6 +271,7 @@ struct RenderArgs {
     uint32_t *out;
     int64_t pitch;
     int32_t ntiles;
+    int32_t chunk_max;     /* slices per chunk, at most kChunk */
 };
 
 #ifndef FS_SQ_LUT
@@ -425,47 +426,51 @@ size_t render_lds_bytes_tc(int32_t rows)
  * Finally the stripe is written row by row with 0 replaced by the sky.  A
  * sample whose colour is 0 is transparent for fill_vline; if one occurs the
  * stripe is instead filled with a top-down scan over all rows. */
-template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
-__global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
+/* Workgroup-shared scalars of one tile render. */
+template <int TC>
+struct TileShared {
+    float sq[256];                 /* squares table, filled once per workgroup */
+    int32_t wtot[kWaves][TC];
+    int32_t wpre[kWaves][TC];
+    int32_t carry[TC];
+    uint32_t qn, nlong, transparent;
+};
+
+template <int TC, int HBITS, bool HP2, bool CP2>
+__device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f, uint32_t *out,
+                                            int64_t pitch, int32_t tile, int32_t chunk_max,
+                                            uint32_t *lds, TileShared<TC> &sh)
 {
     constexpr int SG = kThreads / TC;            /* slice groups */
     constexpr int KR = kChunk / SG;              /* slices per lane per chunk, at most */
-    extern __shared__ uint32_t lds[];
     float4 *tl = (float4 *)lds;                                  /* [kTabStride] */
     float *tz = (float *)(lds + 4 * kTabStride);                 /* [kTabStride] */
-    const int32_t H = a.f.rows, n = a.f.n;
+    const int32_t H = f.rows, n = f.n;
     const int32_t Hp = (H + 3) & ~3;
     uint32_t *stripe = lds + 5 * kTabStride;                     /* [TC][Hp] */
     const int32_t qcap = TC * min(H, kChunk);                    /* visible rows differ */
     uint32_t *queue = stripe + (size_t)Hp * TC;                  /* [qcap] */
     uint16_t *qbot = (uint16_t *)(queue + qcap);                 /* [qcap] span bottoms */
     uint32_t *longs = queue + qcap + (qcap + 1) / 2;             /* [2][TC*(H/24+1)] */
-    __shared__ float sq[256];
-    __shared__ int32_t wtot[kWaves][TC];
-    __shared__ int32_t wpre[kWaves][TC];
-    __shared__ int32_t carry[TC];
-    __shared__ uint32_t qn, nlong, transparent;
 
     const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const int32_t col0 = tile * TC;
     const int32_t c = tid % TC, g = tid / TC;
     const int32_t col = col0 + c;
-    const bool colok = col < a.f.cols;
+    const bool colok = col < f.cols;
     const float fcol = (float)col;
     const float Hf = (float)H;
 
     for (int32_t i = tid; i < Hp * TC; i += kThreads)
         stripe[i] = 0u;
     if (tid < TC)
-        carry[tid] = H;
+        sh.carry[tid] = H;
     if (tid == 0) {
-        nlong = 0u;
-        transparent = 0u;
+        sh.nlong = 0u;
+        sh.transparent = 0u;
     }
-    fill_square_lut(sq, tid);
 
-    const int32_t nch = (n + kChunk - 1) / kChunk;
+    const int32_t nch = (n + chunk_max - 1) / chunk_max;
     const int32_t kr = nch > 0 ? (n + nch * SG - 1) / (nch * SG) : 0;
     const int32_t csz = kr * SG;
 
@@ -473,13 +478,13 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
         const int32_t cn = min(csz, n - base);
         /* phase 0 */
         for (int32_t s = tid; s < cn; s += kThreads) {
-            Slice sl = slice_params(a.f, base + s);
+            Slice sl = slice_params(f, base + s);
             int32_t o = skew(s);
             tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
             tz[o] = sl.iz;
         }
         if (tid == 0)
-            qn = 0u;
+            sh.qn = 0u;
         __syncthreads();
 
         /* phase 1: rows are < 2^16, two per register */
@@ -494,8 +499,8 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
                 float4 L = tl[o];
                 float x = L.x + (fcol * L.z);
                 float y = L.y + (fcol * L.w);
-                float hgt = height_bilinear<HBITS, HP2>(a.m, x, y);
-                float rel = ((a.f.cam_h - hgt) * tz[o]) + a.f.horizon;
+                float hgt = height_bilinear<HBITS, HP2>(m, x, y);
+                float rel = ((f.cam_h - hgt) * tz[o]) + f.horizon;
                 r = screen_row_clamped(rel, Hf);
             }
             if (j & 1)
@@ -517,7 +522,7 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
         if (lane < TC)
             excl = H;
         if (lane >= 64 - TC)
-            wtot[wave][c] = v;
+            sh.wtot[wave][c] = v;
         __syncthreads();
         if (wave == 0) {
             /* lane = w*4 + j handles column j (and j+4 when TC == 8) of
@@ -525,7 +530,7 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
 #pragma unroll
             for (int half = 0; half < TC / 4; half++) {
                 int32_t cc = (lane & 3) + 4 * half;
-                int32_t t = wtot[lane >> 2][cc];
+                int32_t t = sh.wtot[lane >> 2][cc];
 #pragma unroll
                 for (int off = 4; off < 64; off <<= 1) {
                     int32_t o = __shfl_up(t, off);
@@ -533,11 +538,11 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
                         t = min(t, o);
                 }
                 int32_t ex = __shfl_up(t, 4);
-                wpre[lane >> 2][cc] = min(carry[cc], lane < 4 ? H : ex);
+                sh.wpre[lane >> 2][cc] = min(sh.carry[cc], lane < 4 ? H : ex);
             }
         }
         __syncthreads();
-        int32_t run = min(wpre[wave][c], excl);
+        int32_t run = min(sh.wpre[wave][c], excl);
 
         /* phase 3: mark this lane's visible samples, reserve queue slots
          * with one wave-level prefix sum, then queue each sample with the
@@ -564,7 +569,7 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
             uint32_t total = __shfl(incl, 63);
             uint32_t b0 = 0;
             if (lane == 0 && total)
-                b0 = atomicAdd(&qn, total);
+                b0 = atomicAdd(&sh.qn, total);
             b0 = __builtin_amdgcn_readfirstlane(b0);
             uint32_t slot = b0 + incl - cnt;
             int32_t prev = run_in;
@@ -581,17 +586,17 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
             }
         }
         /* the last slice group of each column holds the running minimum of
-         * the whole chunk: the carry into the next chunk */
+         * the whole chunk: the sh.carry into the next chunk */
         if (g == SG - 1)
-            carry[c] = run;
+            sh.carry[c] = run;
         __syncthreads();
         bool done = true;
 #pragma unroll
         for (int j = 0; j < TC; j++)
-            done = done && carry[j] == 0;
+            done = done && sh.carry[j] == 0;
 
         /* phase 4: colour and span */
-        const uint32_t nq = qn;
+        const uint32_t nq = sh.qn;
         for (uint32_t i = tid; i < nq; i += kThreads) {
             uint32_t e = queue[i];
             int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
@@ -600,9 +605,9 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
             float fc = (float)(col0 + cc);
             float x = L.x + (fc * L.z);
             float y = L.y + (fc * L.w);
-            uint32_t colour = color_bilinear_lut<CP2>(a.m, sq, x, y);
+            uint32_t colour = color_bilinear_lut<CP2>(m, sh.sq, x, y);
             if (colour == 0u) {
-                transparent = 1u;
+                sh.transparent = 1u;
                 continue;
             }
             int32_t bottom = (int32_t)qbot[i];
@@ -611,7 +616,7 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
             for (int32_t r = row; r < stop; r++)
                 dst[r] = colour;
             if (bottom > stop) {
-                uint32_t li = atomicAdd(&nlong, 1u);
+                uint32_t li = atomicAdd(&sh.nlong, 1u);
                 longs[2 * li] = ((uint32_t)cc << 26) | ((uint32_t)stop << 13) | (uint32_t)bottom;
                 longs[2 * li + 1] = colour;
             }
@@ -622,7 +627,7 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
     }
 
     /* long spans, painted by the whole workgroup */
-    const uint32_t nl = nlong;
+    const uint32_t nl = sh.nlong;
     for (uint32_t li = 0; li < nl; li++) {
         uint32_t d = longs[2 * li], colour = longs[2 * li + 1];
         int32_t cc = (int32_t)(d >> 26), r0 = (int32_t)((d >> 13) & 0x1FFFu);
@@ -633,9 +638,9 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
     if (nl)
         __syncthreads();
 
-    const bool vec = (col0 + TC <= a.f.cols) && ((a.pitch & 3) == 0) &&
-                     ((((uintptr_t)a.out) & 15u) == 0);
-    if (transparent == 0u) {
+    const bool vec = (col0 + TC <= f.cols) && ((pitch & 3) == 0) &&
+                     ((((uintptr_t)out) & 15u) == 0);
+    if (sh.transparent == 0u) {
         /* phase 5: every non-zero cell is final; 0 is sky.  TC/4 adjacent
          * lanes share a row, so each store instruction writes whole
          * TC*4-byte row segments */
@@ -646,22 +651,22 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 uint32_t s = stripe[(part * 4 + j) * Hp + r];
-                px[j] = s ? s : a.f.sky;
+                px[j] = s ? s : f.sky;
             }
-            uint32_t *dst = a.out + (size_t)r * (size_t)a.pitch + col0 + part * 4;
+            uint32_t *dst = out + (size_t)r * (size_t)pitch + col0 + part * 4;
             if (vec) {
                 *(uint4 *)dst = make_uint4(px[0], px[1], px[2], px[3]);
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    if (col0 + part * 4 + j < a.f.cols)
+                    if (col0 + part * 4 + j < f.cols)
                         dst[j] = px[j];
             }
         }
         return;
     }
 
-    /* phase 5 with transparent samples: a zero cell takes the last
+    /* phase 5 with sh.transparent samples: a zero cell takes the last
      * non-zero colour above it (fill_vline), else the sky; each thread
      * owns a block of consecutive rows */
     const int32_t per = (H + kThreads - 1) / kThreads;
@@ -676,8 +681,8 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
         }
         pre[j] = p;
     }
-    uint32_t *wfill = (uint32_t *)&wtot[0][0];      /* [kWaves][TC] */
-    uint32_t *wfpre = (uint32_t *)&wpre[0][0];
+    uint32_t *wfill = (uint32_t *)&sh.wtot[0][0];      /* [kWaves][TC] */
+    uint32_t *wfpre = (uint32_t *)&sh.wpre[0][0];
 #pragma unroll
     for (int j = 0; j < TC; j++) {
         uint32_t v = pre[j];
@@ -720,9 +725,9 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
         for (int j = 0; j < TC; j++) {
             uint32_t s = stripe[j * Hp + r];
             pre[j] = s ? s : pre[j];
-            px[j] = pre[j] ? pre[j] : a.f.sky;
+            px[j] = pre[j] ? pre[j] : f.sky;
         }
-        uint32_t *dst = a.out + (size_t)r * (size_t)a.pitch + col0;
+        uint32_t *dst = out + (size_t)r * (size_t)pitch + col0;
         if (vec) {
 #pragma unroll
             for (int j = 0; j < TC; j += 4)
@@ -730,10 +735,20 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
         } else {
 #pragma unroll
             for (int j = 0; j < TC; j++)
-                if (col0 + j < a.f.cols)
+                if (col0 + j < f.cols)
                     dst[j] = px[j];
         }
     }
+}
+
+template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+__global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ TileShared<TC> sh;
+    fill_square_lut(sh.sq, threadIdx.x);
+    render_tile<TC, HBITS, HP2, CP2>(a.m, a.f, a.out, a.pitch, xcd_tile(blockIdx.x, a.ntiles),
+                                     a.chunk_max, lds, sh);
 }
 
 /* ------------------------------------------------------------------ */
@@ -882,14 +897,31 @@ hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-/* Render kernel shape: columns per workgroup (8: 32-byte row segments,
- * one 1024-thread workgroup per CU; 4: 16-byte segments, for frames whose
- * 8-column stripe does not fit in LDS) and, for 4 columns, the occupancy
- * the registers are budgeted for (1 or 2 workgroups per CU).
- * FSX_RENDER_TC (4 or 8) and FSX_RENDER_WG_PER_CU (1 or 2) override. */
-struct RenderShape {
-    int tc, wg_per_cu;
-};
+/* ------------------------------------------------------------------ */
+/* Kernel shape and tuning                                             */
+/* ------------------------------------------------------------------ */
+
+/* Render kernel shape: columns per workgroup tile (8: 32-byte row
+ * segments, one 1024-thread workgroup per CU; 4: 16-byte segments, for
+ * frames whose 8-column stripe does not fit in LDS), for 4 columns the
+ * occupancy the registers are budgeted for (1 or 2 workgroups per CU), and
+ * the depth-chunk length.  fsk_set_tuning (the autotuner) or the
+ * FSX_RENDER_TC / FSX_RENDER_WG_PER_CU / FSX_RENDER_CHUNK environment
+ * variables choose them; 0 means automatic. */
+fsk_tuning g_tuning = {-1, -1, -1};
+
+inline fsk_tuning current_tuning()
+{
+    if (g_tuning.tc < 0) {
+        const char *e = getenv("FSX_RENDER_TC");
+        g_tuning.tc = e ? atoi(e) : 0;
+        e = getenv("FSX_RENDER_WG_PER_CU");
+        g_tuning.wg_per_cu = e ? atoi(e) : 0;
+        e = getenv("FSX_RENDER_CHUNK");
+        g_tuning.chunk = e ? atoi(e) : 0;
+    }
+    return g_tuning;
+}
 
 inline size_t lds_per_cu()
 {
@@ -905,34 +937,154 @@ inline size_t lds_per_cu()
     return v;
 }
 
-inline RenderShape render_shape(int32_t rows)
+inline int cu_count()
 {
-    static int env_tc = -1, env_wg = -1;
-    if (env_tc < 0) {
-        const char *e = getenv("FSX_RENDER_TC");
-        env_tc = (e && e[0] == '8') ? 8 : (e && e[0] == '4') ? 4 : 0;
-        e = getenv("FSX_RENDER_WG_PER_CU");
-        env_wg = (e && e[0] == '1') ? 1 : 2;
+    static int v = 0;
+    if (v == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        v = n;
     }
-    RenderShape rs;
-    /* 4 KB of headroom for the kernel's static shared memory */
+    return v;
+}
+
+struct Shape {
+    int tc, wpe, chunk;
+};
+
+inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
+{
+    /* 4 KB of headroom for the kernels' static shared memory */
     bool fits8 = render_lds_bytes_tc<8>(rows) + 4096 <= lds_per_cu();
-    rs.tc = env_tc == 4 ? 4 : (env_tc == 8 || fits8) && fits8 ? 8 : 4;
-    rs.wg_per_cu = rs.tc == 8 ? 1 : env_wg;
-    return rs;
+    Shape s;
+    s.tc = (t.tc == 4 || !fits8) ? 4 : 8;
+    s.wpe = (s.tc == 4 && t.wg_per_cu == 2) ? 8 : 4;
+    s.chunk = (t.chunk >= 256 && t.chunk <= kChunk) ? t.chunk : kChunk;
+    return s;
 }
 
 template <int HBITS, bool HP2, bool CP2>
 hipError_t launch_render(RenderArgs a, hipStream_t s)
 {
-    RenderShape rs = render_shape(a.f.rows);
-    if (rs.tc == 8) {
+    Shape sh = resolve_shape(a.f.rows, current_tuning());
+    a.chunk_max = sh.chunk;
+    if (sh.tc == 8) {
         a.ntiles = (a.f.cols + 7) / 8;
         return launch_render_cfg<8, HBITS, HP2, CP2, 4>(a, s);
     }
     a.ntiles = (a.f.cols + 3) / 4;
-    return rs.wg_per_cu == 1 ? launch_render_cfg<4, HBITS, HP2, CP2, 4>(a, s)
-                             : launch_render_cfg<4, HBITS, HP2, CP2, 8>(a, s);
+    return sh.wpe == 8 ? launch_render_cfg<4, HBITS, HP2, CP2, 8>(a, s)
+                       : launch_render_cfg<4, HBITS, HP2, CP2, 4>(a, s);
+}
+
+/* ------------------------------------------------------------------ */
+/* Batched persistent render                                           */
+/* ------------------------------------------------------------------ */
+
+struct BatchArgs {
+    const fsk_batch_view *views;
+    const uint32_t *items;
+    const uint32_t *qmeta;
+    uint32_t *counters;
+    uint32_t *out;
+    int32_t nqueues, chunk_max;
+};
+
+/* XCD this wave runs on (hardware register XCC_ID). */
+__device__ __forceinline__ uint32_t xcc_id()
+{
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu;
+}
+
+__global__ void k_batch_reset(uint32_t *counters, int32_t n)
+{
+    int32_t i = threadIdx.x;
+    if (i < n)
+        counters[i] = 0u;
+}
+
+/* Persistent workgroups claim (view, tile) items from per-XCD queues: the
+ * host deals each view's tiles out to the queues in contiguous blocks, so
+ * neighbouring tiles of a view stay on one L2; a workgroup whose own queue
+ * is drained takes from the next queues.  Every workgroup exits once all
+ * queues report exhausted, after at most nqueues claims. */
+template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+__global__ __launch_bounds__(kThreads, WPE) void k_render_batch(BatchArgs b)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ TileShared<TC> sh;
+    __shared__ uint32_t cur;
+    fill_square_lut(sh.sq, threadIdx.x);
+    uint32_t q = xcc_id() % (uint32_t)b.nqueues;
+    int32_t tried = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t v = 0xFFFFFFFFu;
+            while (tried < b.nqueues) {
+                uint32_t k = atomicAdd(&b.counters[q], 1u);
+                if (k < b.qmeta[b.nqueues + q]) {
+                    v = b.items[b.qmeta[q] + k];
+                    break;
+                }
+                tried++;
+                q = (q + 1u) % (uint32_t)b.nqueues;
+            }
+            cur = v;
+        }
+        __syncthreads();
+        uint32_t it = cur;
+        __syncthreads();
+        if (it == 0xFFFFFFFFu)
+            break;
+        const fsk_batch_view &bv = b.views[it >> 16];
+        render_tile<TC, HBITS, HP2, CP2>(bv.m, bv.f, b.out + bv.offset, bv.f.cols, it & 0xFFFFu,
+                                         b.chunk_max, lds, sh);
+    }
+}
+
+template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
+{
+    auto kern = k_render_batch<TC, HBITS, HP2, CP2, WPE>;
+    static int dyn_max = -1;
+    if (dyn_max < 0) {
+        hipFuncAttributes fa;
+        hipError_t e = hipFuncGetAttributes(&fa, (const void *)kern);
+        if (e != hipSuccess)
+            return e;
+        int want = (int)lds_per_cu() - (int)fa.sharedSizeBytes;
+        e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+        if (e != hipSuccess)
+            return e;
+        dyn_max = want;
+    }
+    size_t need = render_lds_bytes_tc<TC>(fb->max_rows);
+    if (need > (size_t)dyn_max)
+        return hipErrorInvalidValue;
+    BatchArgs b;
+    b.views = fb->views;
+    b.items = fb->items;
+    b.qmeta = fb->qmeta;
+    b.counters = fb->counters;
+    b.out = fb->out;
+    b.nqueues = fb->nqueues;
+    b.chunk_max = chunk;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, fb->counters, fb->nqueues);
+    hipLaunchKernelGGL(kern, dim3(fb->grid), dim3(kThreads), need, s, b);
+    return hipGetLastError();
+}
+
+template <int HBITS, bool HP2, bool CP2>
+hipError_t launch_batch(const fsk_batch *fb, hipStream_t s)
+{
+    Shape sh = resolve_shape(fb->max_rows, fb->tune);
+    if (sh.tc == 8)
+        return launch_batch_cfg<8, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
+    return sh.wpe == 8 ? launch_batch_cfg<4, HBITS, HP2, CP2, 8>(fb, sh.chunk, s)
+                       : launch_batch_cfg<4, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
 }
 
 inline bool pow2(int32_t v)
@@ -946,7 +1098,33 @@ extern "C" {
 
 size_t fsk_render_lds_bytes(int32_t rows)
 {
-    return render_shape(rows).tc == 8 ? render_lds_bytes_tc<8>(rows) : render_lds_bytes_tc<4>(rows);
+    return resolve_shape(rows, current_tuning()).tc == 8 ? render_lds_bytes_tc<8>(rows)
+                                                         : render_lds_bytes_tc<4>(rows);
+}
+
+int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t)
+{
+    return resolve_shape(max_rows, *t).tc;
+}
+
+int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t)
+{
+    Shape sh = resolve_shape(max_rows, *t);
+    return cu_count() * (sh.wpe == 8 ? 2 : 1);
+}
+
+void fsk_set_tuning(const fsk_tuning *t)
+{
+    g_tuning = *t;
+}
+
+int fsk_batch_run(const fsk_batch *b, hipStream_t s)
+{
+    if (b->max_rows > kMaxRows || b->nqueues < 1 || b->nqueues > 64)
+        return hipErrorInvalidValue;
+    if (b->hbits == 8)
+        return b->pow2 ? launch_batch<8, true, true>(b, s) : launch_batch<8, false, false>(b, s);
+    return b->pow2 ? launch_batch<16, true, true>(b, s) : launch_batch<16, false, false>(b, s);
 }
 
 int32_t fsk_render_max_rows(void)
@@ -987,6 +1165,7 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pi
     a.out = d_out;
     a.pitch = pitch;
     a.ntiles = 0;
+    a.chunk_max = kChunk;
     bool hp2 = pow2(m->hrows) && pow2(m->hcols);
     bool cp2 = pow2(m->crows) && pow2(m->ccols);
     hipError_t e;

@@ -138,6 +138,66 @@ class Context:
                    "fsx_render_batch")
         return out
 
+    def batch(self, maps, views, out, offsets):
+        """A prepared batch that can be run, captured into a HIP graph and
+        replayed (fsx_batch_*)."""
+        return Batch(self, maps, views, out, offsets)
+
+    def set_tuning(self, tile_columns=0, wg_per_cu=0, chunk=0):
+        """Render kernel shape (0 = automatic); see fsx_set_tuning."""
+        self.check(self.lib.fsx_set_tuning(self.ptr, tile_columns, wg_per_cu, chunk),
+                   "fsx_set_tuning")
+
+    def profile(self, enable=True):
+        self.check(self.lib.fsx_profile_enable(self.ptr, 1 if enable else 0),
+                   "fsx_profile_enable")
+
+    def profile_read(self):
+        """(summed kernel milliseconds, launches) since the last read."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self.check(self.lib.fsx_profile_read(self.ptr, ctypes.byref(ms), ctypes.byref(n)),
+                   "fsx_profile_read")
+        return ms.value, n.value
+
+
+class Batch:
+    """fsx_batch: many views over resident maps in one persistent launch."""
+
+    def __init__(self, ctx, maps, views, out, offsets):
+        self.ctx = ctx
+        self.maps = list(maps)           # keep the maps alive with the batch
+        self.out = out
+        n = len(views)
+        cviews = (FsxView * n)(*[v.to_c() for v in views])
+        cmaps = (ctypes.c_void_p * len(maps))(*[m.ptr for m in maps])
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.fsx_batch_new(ctx.ptr, cmaps, len(maps), cviews, n,
+                                        ctypes.c_void_p(out.data_ptr()), offs.ctypes.data,
+                                        ctypes.byref(p)), "fsx_batch_new")
+        self.ptr = p.value
+
+    def run(self):
+        self.ctx.check(self.ctx.lib.fsx_batch_run(self.ctx.ptr, self.ptr), "fsx_batch_run")
+
+    def capture(self):
+        self.ctx.check(self.ctx.lib.fsx_batch_capture(self.ctx.ptr, self.ptr),
+                       "fsx_batch_capture")
+
+    def replay(self):
+        self.ctx.check(self.ctx.lib.fsx_batch_replay(self.ctx.ptr, self.ptr), "fsx_batch_replay")
+
+    def close(self):
+        if self.ptr and self.ctx.ptr:
+            self.ctx.lib.fsx_batch_free(self.ctx.ptr, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class Map:
     """A resident map pair with its shaded colour map (extension entry)."""

@@ -136,6 +136,12 @@ BATCH_MAP_SEEDS = (3, 4, 5, 6)
 BATCH_RESOLUTIONS = ((240, 320), (480, 640), (720, 1280), (1080, 1920))
 
 
+def batch_maps():
+    """The batched workload's four (colour, height) maps: 1024^2, 2048^2,
+    2048^2 and 4096^2, seeds 3-6."""
+    return [scene_maps(s, n) for s, n in zip(BATCH_MAP_SEEDS, BATCH_MAP_SIZES)]
+
+
 def batched_views(heights_list, n_views=256, seed=7):
     """heights_list: the (masked) heightmaps of the batch's maps."""
     map_sizes = [h.shape[0] for h in heights_list]

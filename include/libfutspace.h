@@ -161,10 +161,31 @@ int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,
 
 /* Batched render: views[i] draws maps[views[i].map] into dev_out +
  * offsets[i] (u32 elements, row pitch views[i].w).  One persistent
- * launch covers all views. */
+ * launch covers all views; per-XCD work queues hand out column tiles. */
 int fsx_render_batch(struct futhark_context *ctx, const struct fsx_map *const *maps,
                      int32_t n_maps, const struct fsx_view *views, int32_t n_views,
                      uint32_t *dev_out, const int64_t *offsets);
+
+/* A prepared batch (same arguments as fsx_render_batch), launched any
+ * number of times.  fsx_batch_run enqueues it on the context stream;
+ * fsx_batch_capture records that launch into a HIP graph once and
+ * fsx_batch_replay relaunches the graph.  The batch holds references to
+ * its maps; the output buffer must outlive it. */
+struct fsx_batch;
+int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps, int32_t n_maps,
+                  const struct fsx_view *views, int32_t n_views, uint32_t *dev_out,
+                  const int64_t *offsets, struct fsx_batch **out);
+int fsx_batch_run(struct futhark_context *ctx, struct fsx_batch *batch);
+int fsx_batch_capture(struct futhark_context *ctx, struct fsx_batch *batch);
+int fsx_batch_replay(struct futhark_context *ctx, struct fsx_batch *batch);
+int fsx_batch_free(struct futhark_context *ctx, struct fsx_batch *batch);
+
+/* Render kernel shape, as chosen by the autotuner: columns per workgroup
+ * tile (4 or 8), workgroups per CU for 4-column tiles (1 or 2) and depth
+ * slices per chunk (256-2048); 0 leaves a knob automatic.  Applies to
+ * later renders and to batches prepared afterwards. */
+int fsx_set_tuning(struct futhark_context *ctx, int32_t tile_columns, int32_t wg_per_cu,
+                   int32_t chunk);
 
 /* Use `stream` (a hipStream_t, or NULL for the context's own stream) for
  * all later device work of the context. */

@@ -153,27 +153,6 @@ def test_render_into_torch_tensor(ctx, small_scene):
     assert not got[:, 128:].any(), "render wrote past the frame width"
 
 
-def test_batch_matches_single_views(ctx):
-    torch = pytest.importorskip("torch")
-    maps_np = [scene.scene_maps(s, n) for s, n in ((3, 256), (4, 128))]
-    maps = [ctx.map(c, h) for c, h in maps_np]
-    views = []
-    for i in range(9):
-        views.append(View(x=17.0 * i + 0.3, y=11.0 * i + 0.7, height=300.0, angle=0.7 * i,
-                          horizon=100.0, distance=500.0, fov=1.2, sky_color=sky_color(0.1),
-                          h=48 + 16 * (i % 3), w=64 + 8 * i, d=0.004, z0=0.5, map=i % 2))
-    offs, total = scene.batch_offsets(views)
-    out = torch.zeros(total, dtype=torch.int32, device="cuda")
-    ctx.set_stream(torch.cuda.current_stream())
-    ctx.render_batch(maps, views, out, offs)
-    torch.cuda.synchronize()
-    ctx.set_stream(None)
-    flat = out.cpu().numpy().view(np.uint32)
-    for v, o in zip(views, offs):
-        got = flat[o:o + v.h * v.w].reshape(v.h, v.w)
-        _check(got, ctx.render(maps[v.map], v))
-
-
 @pytest.mark.parametrize("which", [0, 1, 2, 3])
 def test_device_arithmetic_selftest(ctx, which):
     """The kernels' shortcuts (byte/255 by reciprocal + FMA correction, sqrt
