@@ -30,6 +30,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--quick", action="store_true", help="few steps, for profiling runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-view leg")
+    ap.add_argument("--batch-reps", type=int, default=20,
+                    help="timed graph replays of the 256-view batch")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 64 MiB device copy (known bytes) to calibrate HBM counters")
     return ap.parse_args()
@@ -44,6 +47,72 @@ def calib_copy():
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
+
+
+def bench_batched(ctx, reps):
+    """256 views of mixed resolution and depth over four resident maps:
+    one persistent launch captured in a HIP graph (shape autotuned on this
+    workload), against a loop of plain renders of the same views."""
+    import numpy as np
+    import torch
+
+    from futspace_amd import autotune, scene
+
+    data = scene.batch_maps()
+    maps = [ctx.map(c, h) for c, h in data]
+    views = scene.batched_views([h for _, h in data])
+    offs, total = scene.batch_offsets(views)
+    out = torch.zeros(total, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+
+    def make_run():
+        b = ctx.batch(maps, views, out, offs)
+        return b.run
+
+    best, timings = autotune.tune(ctx, make_run, "batched-256", reps=3, use_cache=False)
+    b = ctx.batch(maps, views, out, offs)
+    b.capture()
+    for _ in range(3):
+        b.replay()
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.replay()
+    ctx.sync()
+    t1 = time.perf_counter()
+    graph_ms, launches = ctx.profile_read()
+    ctx.profile(False)
+    batch_ms = 1e3 * (t1 - t0) / reps
+
+    # the same views as a loop of plain renders (automatic shape)
+    ctx.set_tuning(0, 0, 0)
+    slices = [out[o:o + v.h * v.w].view(v.h, v.w) for v, o in zip(views, offs)]
+    loop_reps = max(2, reps // 4)
+    for v, sl in zip(views, slices):
+        ctx.render(maps[v.map], v, sl)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(loop_reps):
+        for v, sl in zip(views, slices):
+            ctx.render(maps[v.map], v, sl)
+    ctx.sync()
+    t1 = time.perf_counter()
+    loop_ms = 1e3 * (t1 - t0) / loop_reps
+    b.close()
+    pixels = int(sum(v.h * v.w for v in views))
+    alg = float(sum(scene.algorithmic_bytes(v, 1) for v in views))
+    kern_ms = graph_ms / max(launches, 1)
+    return {
+        "views": len(views), "maps": list(scene.BATCH_MAP_SIZES), "pixels": pixels,
+        "ms_per_batch": batch_ms, "graph_kernel_ms": kern_ms,
+        "views_per_s": len(views) / (batch_ms * 1e-3),
+        "gpix_per_s": pixels / (batch_ms * 1e-3) / 1e9,
+        "achieved_gbs": alg / (kern_ms * 1e-3) / 1e9, "algorithmic_bytes": alg,
+        "plain_loop_ms": loop_ms, "speedup_vs_plain_loop": loop_ms / batch_ms,
+        "tuning": {"tile_columns": best[0], "wg_per_cu": best[1], "chunk": best[2]},
+        "autotune_ms": {"x".join(map(str, k)): round(v, 4) for k, v in timings.items()},
+    }
 
 
 def cpu_baseline(views, shaded, alt, budget_s):
@@ -147,6 +216,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_render", "algorithmic_bytes_per_launch": bytes_per_frame},
     }
+    if not args.no_batch:
+        ctx.set_stream(None)
+        rec["batched"] = bench_batched(ctx, max(3, args.batch_reps if not args.quick else 3))
     if args.calib_copy:
         calib_copy()
     if not args.no_cpu:

@@ -109,6 +109,11 @@ FUTSPACE_SIGNATURES = {
     "fsx_batch_free": (ctypes.c_int, [P, P]),
     "fsx_set_tuning": (ctypes.c_int, [P, I32, I32, I32]),
     "fsx_context_set_stream": (ctypes.c_int, [P, P]),
+    "fsx_context_stream": (P, [P]),
+    "fsx_map_shade_factor": (ctypes.c_int, [P, P, P]),
+    "fsx_render_f32": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, P, I32, I32, F32, P, P]),
+    "fsx_render_f32_vjp": (ctypes.c_int, [P, ctypes.POINTER(FsxView), P, P, I32, I32, P, P, I32,
+                                          P, P]),
     "fsx_context_device": (ctypes.c_int, [P]),
     "fsx_version": (ctypes.c_char_p, []),
     "fsx_sky": (U32, [F32]),

@@ -45,8 +45,6 @@ struct futhark_context {
     hipStream_t own_stream;
     hipStream_t stream;
     char *error;
-    float *d_tab;             /* slice table */
-    size_t tab_cap;           /* floats */
     struct {
         void *ptr;
         size_t bytes;
@@ -270,8 +268,6 @@ void futhark_context_free(struct futhark_context *ctx)
     for (int i = 0; i < FS_POOL_SLOTS; i++)
         if (ctx->pool[i].ptr)
             hipFree(ctx->pool[i].ptr);
-    if (ctx->d_tab)
-        hipFree(ctx->d_tab);
     if (ctx->prof_ev) {
         for (int i = 0; i < 2 * FS_PROF_MAX; i++)
             if (ctx->prof_ev[i])
@@ -328,16 +324,24 @@ void futhark_context_unpause_profiling(struct futhark_context *ctx)
 void futhark_debugging_report(struct futhark_context *ctx)
 {
     if (ctx && ctx->debugging)
-        fprintf(stderr, "futspace: device %d, slice table %zu floats\n", ctx->device, ctx->tab_cap);
+        fprintf(stderr, "futspace: device %d, %s\n", ctx->device, FS_VERSION);
 }
 
 int fsx_context_set_stream(struct futhark_context *ctx, void *stream)
 {
     if (!ctx)
         return 1;
+    hipStream_t next = stream ? (hipStream_t)stream : ctx->own_stream;
+    if (next == ctx->stream)
+        return 0;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    ctx->stream = next;
     return 0;
+}
+
+void *fsx_context_stream(struct futhark_context *ctx)
+{
+    return ctx ? (void *)ctx->stream : NULL;
 }
 
 int fsx_context_device(struct futhark_context *ctx)
@@ -590,6 +594,7 @@ struct fs_terrain {
 struct fs_shade {
     int refs;
     int32_t rows, cols;
+    float sun[3];          /* the sun vector it was shaded with */
     uint32_t *d_shaded;
     void *d_cq;            /* bilinear colour quads (uint4 per texel) */
 };
@@ -701,6 +706,9 @@ static struct fs_shade *shade_new(struct futhark_context *ctx, const struct fs_t
     float sun0 = ca * v0;
     float sun1 = v1;
     float sun2 = (-sa) * v0;
+    s->sun[0] = sun0;
+    s->sun[1] = sun1;
+    s->sun[2] = sun2;
     e = (hipError_t)fsk_shade(t->d_hp, t->rows, t->cols, t->d_color, t->rows, t->cols,
                               sun0, sun1, sun2, size, s->d_shaded, ctx->stream);
     if (e == hipSuccess)
@@ -787,23 +795,6 @@ static void frame_of_view(const struct fsx_view *v, fsk_frame *f)
     f->rows = v->h;
     f->cols = v->w;
     f->sky = v->sky_color;
-}
-
-static int ensure_tab(struct futhark_context *ctx, int32_t n)
-{
-    size_t need = (size_t)(n > 0 ? n : 1) * 5;
-    if (need <= ctx->tab_cap)
-        return 0;
-    if (ctx->d_tab) {
-        HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-        hipFree(ctx->d_tab);
-        ctx->d_tab = NULL;
-        ctx->tab_cap = 0;
-    }
-    size_t cap = need < 8192 * 5 ? 8192 * 5 : need;
-    HIPCHECK(ctx, hipMalloc((void **)&ctx->d_tab, cap * sizeof(float)));
-    ctx->tab_cap = cap;
-    return 0;
 }
 
 static int render_into(struct futhark_context *ctx, const struct fs_terrain *t,
@@ -1345,6 +1336,97 @@ int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,
     }
     dev_release(ctx, d, bytes);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Differentiable renderer                                             */
+/* ------------------------------------------------------------------ */
+
+int fsx_map_shade_factor(struct futhark_context *ctx, const struct fsx_map *map, float *dev_sf)
+{
+    if (!ctx || !map || !dev_sf)
+        return 1;
+    const struct fs_terrain *t = map->t;
+    const struct fs_shade *s = map->s;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    HIPCHECK(ctx, fsk_shade_factor(t->d_hp, t->rows, t->cols, s->sun[0], s->sun[1], s->sun[2],
+                                   s->rows, dev_sf, ctx->stream));
+    return 0;
+}
+
+static int diff_frame(struct futhark_context *ctx, const char *what, const struct fsx_view *v,
+                      int32_t hc, int32_t wc, fsk_frame *f)
+{
+    if (v->h < 0 || v->w < 0 || v->h > fsk_render_max_rows()) {
+        set_error(ctx, "%s: unsupported frame size %d x %d", what, v->h, v->w);
+        return 1;
+    }
+    if (hc <= 0 || wc <= 0) {
+        set_error(ctx, "%s: colour planes of %d x %d", what, hc, wc);
+        return 1;
+    }
+    frame_of_view(v, f);
+    return 0;
+}
+
+int fsx_render_f32(struct futhark_context *ctx, const struct fsx_map *map,
+                   const struct fsx_view *view, const float *dev_planes, const float *dev_sf,
+                   int32_t hc, int32_t wc, float sky_scale, float *dev_img, int32_t *dev_src)
+{
+    if (!ctx || !map || !view)
+        return 1;
+    fsk_frame f;
+    if (diff_frame(ctx, "fsx_render_f32", view, hc, wc, &f))
+        return 1;
+    if (view->h == 0 || view->w == 0)
+        return 0;
+    if (!dev_planes || !dev_sf || !dev_img || !dev_src) {
+        set_error(ctx, "fsx_render_f32: null buffer");
+        return 1;
+    }
+    fsk_map m;
+    m.hq = map->t->d_hq;
+    m.hrows = map->t->rows;
+    m.hcols = map->t->cols;
+    m.hbits = map->t->hbits;
+    m.cq = NULL;
+    m.crows = hc;
+    m.ccols = wc;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int pe = prof_begin(ctx);
+    HIPCHECK(ctx, fsk_render_slices(&m, &f, dev_src, view->w, ctx->stream));
+    HIPCHECK(ctx, fsk_diff_color(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_img,
+                                 ctx->stream));
+    prof_end(ctx, pe);
+    return 0;
+}
+
+int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
+                       const float *dev_planes, const float *dev_sf, int32_t hc, int32_t wc,
+                       const int32_t *dev_src, const float *dev_gimg, int32_t accumulate,
+                       float *dev_gplanes, float *dev_gsky)
+{
+    if (!ctx || !view)
+        return 1;
+    fsk_frame f;
+    if (diff_frame(ctx, "fsx_render_f32_vjp", view, hc, wc, &f))
+        return 1;
+    if (!dev_planes || !dev_sf || !dev_gplanes || !dev_gsky ||
+        ((!dev_src || !dev_gimg) && view->h > 0 && view->w > 0)) {
+        set_error(ctx, "fsx_render_f32_vjp: null buffer");
+        return 1;
+    }
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int pe = prof_begin(ctx);
+    if (!accumulate) {
+        HIPCHECK(ctx, fsk_zero_f32(dev_gplanes, (size_t)3 * (size_t)hc * (size_t)wc, ctx->stream));
+        HIPCHECK(ctx, fsk_zero_f32(dev_gsky, 1, ctx->stream));
+    }
+    if (view->h > 0 && view->w > 0)
+        HIPCHECK(ctx, fsk_diff_grad(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg,
+                                    dev_gplanes, dev_gsky, ctx->stream));
+    prof_end(ctx, pe);
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

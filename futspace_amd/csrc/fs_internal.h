@@ -99,6 +99,22 @@ int fsk_shade(const uint16_t *d_hplain, int32_t hrows, int32_t hcols,
 int fsk_color_quads(const uint32_t *d_col, int32_t rows, int32_t cols, void *d_cq,
                     hipStream_t s);
 
+/* Differentiable renderer (f32 colour planes, visibility held fixed).
+ * fsk_render_slices writes, per pixel, the depth slice whose sample shows
+ * there (-1: sky); fsk_diff_color turns it into img[3][rows][cols];
+ * fsk_diff_grad adds dL/dcp into d_gcp[3][hc][wc] and dL/dsky_scale into
+ * *d_gsky (both accumulate). */
+int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
+                      hipStream_t s);
+int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                   int32_t wc, float sky_scale, const int32_t *d_src, float *d_img, hipStream_t s);
+int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                  int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
+                  float *d_gsky, hipStream_t s);
+int fsk_zero_f32(float *d_p, size_t n, hipStream_t s);
+int fsk_shade_factor(const uint16_t *d_hplain, int32_t hrows, int32_t hcols, float sun0,
+                     float sun1, float sun2, int32_t size, float *d_sf, hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -436,7 +436,23 @@ struct TileShared {
     uint32_t qn, nlong, transparent;
 };
 
-template <int TC, int HBITS, bool HP2, bool CP2>
+/* What a tile writes per pixel: the u32 colour (the reference frame) or,
+ * for the differentiable renderer, the index of the depth slice whose
+ * sample shows there (-1 for sky).  The slice-index mode keeps the same
+ * visibility and fill rules; a sample is transparent there iff its map
+ * coordinate is integral in x or y, which is where argb.mix's weight sum
+ * vanishes. */
+enum { kOutColour = 0, kOutSlice = 1 };
+
+template <int MODE>
+__device__ __forceinline__ uint32_t tile_value(uint32_t v, uint32_t sky)
+{
+    if (MODE == kOutColour)
+        return v ? v : sky;
+    return v ? v - 1u : 0xFFFFFFFFu;
+}
+
+template <int TC, int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
 __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f, uint32_t *out,
                                             int64_t pitch, int32_t tile, int32_t chunk_max,
                                             uint32_t *lds, TileShared<TC> &sh)
@@ -605,7 +621,12 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
             float fc = (float)(col0 + cc);
             float x = L.x + (fc * L.z);
             float y = L.y + (fc * L.w);
-            uint32_t colour = color_bilinear_lut<CP2>(m, sh.sq, x, y);
+            uint32_t colour;
+            if (MODE == kOutColour)
+                colour = color_bilinear_lut<CP2>(m, sh.sq, x, y);
+            else
+                colour = (floorf(x) == ceilf(x) || floorf(y) == ceilf(y))
+                             ? 0u : (uint32_t)(base + s + 1);
             if (colour == 0u) {
                 sh.transparent = 1u;
                 continue;
@@ -651,7 +672,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 uint32_t s = stripe[(part * 4 + j) * Hp + r];
-                px[j] = s ? s : f.sky;
+                px[j] = tile_value<MODE>(s, f.sky);
             }
             uint32_t *dst = out + (size_t)r * (size_t)pitch + col0 + part * 4;
             if (vec) {
@@ -725,7 +746,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         for (int j = 0; j < TC; j++) {
             uint32_t s = stripe[j * Hp + r];
             pre[j] = s ? s : pre[j];
-            px[j] = pre[j] ? pre[j] : f.sky;
+            px[j] = tile_value<MODE>(pre[j], f.sky);
         }
         uint32_t *dst = out + (size_t)r * (size_t)pitch + col0;
         if (vec) {
@@ -741,14 +762,14 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     }
 }
 
-template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+template <int TC, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
 __global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
 {
     extern __shared__ uint32_t lds[];
     __shared__ TileShared<TC> sh;
     fill_square_lut(sh.sq, threadIdx.x);
-    render_tile<TC, HBITS, HP2, CP2>(a.m, a.f, a.out, a.pitch, xcd_tile(blockIdx.x, a.ntiles),
-                                     a.chunk_max, lds, sh);
+    render_tile<TC, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
+                                           xcd_tile(blockIdx.x, a.ntiles), a.chunk_max, lds, sh);
 }
 
 /* ------------------------------------------------------------------ */
@@ -868,10 +889,10 @@ __global__ void k_selftest(int which, unsigned long long *bad)
         atomicAdd(bad, local);
 }
 
-template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+template <int TC, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
 hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
 {
-    auto kern = k_render<TC, HBITS, HP2, CP2, WPE>;
+    auto kern = k_render<TC, HBITS, HP2, CP2, WPE, MODE>;
     static int dyn_max = -1;
     if (dyn_max < 0) {
         /* the whole LDS of a CU minus the kernel's static shared memory */
@@ -966,18 +987,18 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
     return s;
 }
 
-template <int HBITS, bool HP2, bool CP2>
+template <int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
 hipError_t launch_render(RenderArgs a, hipStream_t s)
 {
     Shape sh = resolve_shape(a.f.rows, current_tuning());
     a.chunk_max = sh.chunk;
     if (sh.tc == 8) {
         a.ntiles = (a.f.cols + 7) / 8;
-        return launch_render_cfg<8, HBITS, HP2, CP2, 4>(a, s);
+        return launch_render_cfg<8, HBITS, HP2, CP2, 4, MODE>(a, s);
     }
     a.ntiles = (a.f.cols + 3) / 4;
-    return sh.wpe == 8 ? launch_render_cfg<4, HBITS, HP2, CP2, 8>(a, s)
-                       : launch_render_cfg<4, HBITS, HP2, CP2, 4>(a, s);
+    return sh.wpe == 8 ? launch_render_cfg<4, HBITS, HP2, CP2, 8, MODE>(a, s)
+                       : launch_render_cfg<4, HBITS, HP2, CP2, 4, MODE>(a, s);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1087,6 +1108,223 @@ hipError_t launch_batch(const fsk_batch *fb, hipStream_t s)
                        : launch_batch_cfg<4, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
 }
 
+/* ------------------------------------------------------------------ */
+/* Differentiable renderer: f32 colour planes                           */
+/* ------------------------------------------------------------------ */
+
+/* The f32 relaxation of the colour path: planes cp[3][hc][wc] in [0,1]
+ * replace the u32 ARGB map, each texel scaled by its shade factor
+ * sf = sqrt(1/(4*count + 1)) (argb.mix(4*count, black, 1, c) without
+ * quantisation, fut/effects.fut:123), and argb.mix's channel rule
+ * sqrt(m1'*a^2 + m2'*b^2) without u8 rounding (matte colour.fut, as
+ * png_color_filtered calls it at fut/render_functions.fut:101-103).
+ * Visibility is the integer renderer's, held fixed: src[r][c] is the
+ * depth slice whose sample shows at the pixel, -1 for sky.  Sky pixels
+ * are sky_base * sky_scale with sky_base = (0x90, 0x90, 0xE0) / 255
+ * (argb.scale of 0xFF9090e0, fut/interactive.fut:163). */
+
+constexpr int kDiffRows = 16;          /* rows one thread walks down a column */
+
+struct DiffArgs {
+    fsk_frame f;
+    const float *cp;
+    const float *sf;
+    int32_t hc, wc;
+    float sky_scale;
+    const int32_t *src;
+    const float *gimg;     /* backward: dL/dimg [3][rows][cols] */
+    float *img;            /* forward output [3][rows][cols] */
+    float *gcp;            /* backward: dL/dcp accumulated [3][hc][wc] */
+    float *gsky;           /* backward: dL/dsky_scale accumulated */
+};
+
+__device__ __forceinline__ float mixf(float m1, float a, float m2, float b)
+{
+    float m12 = m1 + m2;
+    float m1n = m1 / m12, m2n = m2 / m12;
+    return __builtin_sqrtf((m1n * (a * a)) + (m2n * (b * b)));
+}
+
+/* One visible sample: texel offsets (fy,fx) (fy,cx) (cy,fx) (cy,cx) and
+ * the bilinear weights, as png_color_filtered forms them
+ * (fut/render_functions.fut:95-105) with floor-mod indices. */
+struct Bilin {
+    size_t t[4];
+    float wx0, wx1, wy0, wy1;
+};
+
+__device__ __forceinline__ Bilin bilin_of(const fsk_frame &f, int32_t k, int32_t col, int32_t hc,
+                                          int32_t wc)
+{
+    Slice sl = slice_params(f, k);
+    float fcol = (float)col;
+    float x = sl.sx + (fcol * sl.dx);
+    float y = sl.sy + (fcol * sl.dy);
+    float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
+    int32_t ify = fmod_i32(i32_f32_x86(fy), hc), icy = fmod_i32(i32_f32_x86(cy), hc);
+    int32_t ifx = fmod_i32(i32_f32_x86(fx), wc), icx = fmod_i32(i32_f32_x86(cx), wc);
+    Bilin b;
+    b.t[0] = (size_t)ify * wc + ifx;
+    b.t[1] = (size_t)ify * wc + icx;
+    b.t[2] = (size_t)icy * wc + ifx;
+    b.t[3] = (size_t)icy * wc + icx;
+    b.wx0 = cx - x;
+    b.wx1 = x - fx;
+    b.wy0 = cy - y;
+    b.wy1 = y - fy;
+    return b;
+}
+
+__device__ __forceinline__ float sky_base(int ch)
+{
+    return ch == 2 ? 224.0f / 255.0f : 144.0f / 255.0f;
+}
+
+/* Forward colour: each thread walks kDiffRows rows of one column and
+ * evaluates a sample once per run of equal src. */
+__global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
+{
+    const int32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r0 = blockIdx.y * kDiffRows;
+    const int32_t H = a.f.rows, W = a.f.cols;
+    if (col >= W)
+        return;
+    const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
+    int32_t last = -2;
+    float px[3] = {0.0f, 0.0f, 0.0f};
+    for (int32_t r = r0; r < min(r0 + kDiffRows, H); r++) {
+        const size_t p = (size_t)r * W + col;
+        const int32_t k = a.src[p];
+        if (k != last) {
+            last = k;
+            if (k < 0) {
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++)
+                    px[ch] = sky_base(ch) * a.sky_scale;
+            } else {
+                Bilin b = bilin_of(a.f, k, col, a.hc, a.wc);
+                float sf[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    sf[t] = a.sf[b.t[t]];
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    const float *P = a.cp + ch * plane;
+                    float s0 = P[b.t[0]] * sf[0], s1 = P[b.t[1]] * sf[1];
+                    float s2 = P[b.t[2]] * sf[2], s3 = P[b.t[3]] * sf[3];
+                    float X1 = mixf(b.wx0, s0, b.wx1, s1);
+                    float X2 = mixf(b.wx0, s2, b.wx1, s3);
+                    px[ch] = mixf(b.wy0, X1, b.wy1, X2);
+                }
+            }
+        }
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++)
+            a.img[ch * fplane + p] = px[ch];
+    }
+}
+
+/* Adjoint: per run of equal src the incoming gradient is summed, then
+ * pushed through the three mixes once.  For px = mix(wy0, X1, wy1, X2),
+ * X1 = mix(wx0, s0, wx1, s1): d px / d s0 = wy0' * wx0' * s0 / px (X1
+ * cancels), and likewise for the other texels; 0 where px == 0.  The
+ * texel gradients (times sf, d s / d cp) are scatter-added with f32
+ * atomics; the sky term is reduced per wave. */
+__global__ __launch_bounds__(256) void k_diff_grad(DiffArgs a)
+{
+    const int32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r0 = blockIdx.y * kDiffRows;
+    const int32_t H = a.f.rows, W = a.f.cols;
+    float sky_acc = 0.0f;
+    if (col < W) {
+        const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
+        const int32_t r1 = min(r0 + kDiffRows, H);
+        int32_t r = r0;
+        while (r < r1) {
+            const int32_t k = a.src[(size_t)r * W + col];
+            float g[3] = {0.0f, 0.0f, 0.0f};
+            int32_t e = r;
+            for (; e < r1 && a.src[(size_t)e * W + col] == k; e++) {
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++)
+                    g[ch] += a.gimg[ch * fplane + (size_t)e * W + col];
+            }
+            r = e;
+            if (k < 0) {
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++)
+                    sky_acc += g[ch] * sky_base(ch);
+                continue;
+            }
+            Bilin b = bilin_of(a.f, k, col, a.hc, a.wc);
+            float sf[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                sf[t] = a.sf[b.t[t]];
+            const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
+            const float wx0n = b.wx0 / wxs, wx1n = b.wx1 / wxs;
+            const float wy0n = b.wy0 / wys, wy1n = b.wy1 / wys;
+            const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) {
+                const float *P = a.cp + ch * plane;
+                float s[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    s[t] = P[b.t[t]] * sf[t];
+                float X1 = mixf(b.wx0, s[0], b.wx1, s[1]);
+                float X2 = mixf(b.wx0, s[2], b.wx1, s[3]);
+                float px = mixf(b.wy0, X1, b.wy1, X2);
+                if (!(px > 0.0f))
+                    continue;
+                float q = g[ch] / px;
+                float *G = a.gcp + ch * plane;
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    unsafeAtomicAdd(&G[b.t[t]], ((q * wt[t]) * s[t]) * sf[t]);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        sky_acc += __shfl_xor(sky_acc, off);
+    if ((threadIdx.x & 63) == 0 && sky_acc != 0.0f)
+        unsafeAtomicAdd(a.gsky, sky_acc);
+}
+
+/* Zero fill (a kernel, not a memset node, so that callers may capture it). */
+__global__ void k_zero_f32(float *__restrict__ p, size_t n)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = 0.0f;
+}
+
+/* Shade factor of the relaxation: the same sun-ray count as k_shade,
+ * then sf = sqrt(1 / (4*count + 1)). */
+__global__ void k_shade_factor(const uint16_t *__restrict__ hp, int32_t hrows, int32_t hcols,
+                               float s0, float s1, float s2, int32_t size, float *__restrict__ sf)
+{
+    int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t y = blockIdx.y;
+    if (x >= size)
+        return;
+    const float step_size = 4.0f;
+    float fx = (float)x, fy = (float)y;
+    float height = (float)hp[(size_t)fmod_i32(y, hrows) * hcols + fmod_i32(x, hcols)];
+    int32_t count = 0;
+    for (int32_t dist = 1; dist < 256; dist++) {
+        float t = (float)dist * step_size;
+        float px = fx + (t * s0);
+        float py = fy + (t * s2);
+        int32_t ix = fmod_i32(i32_f32_x86(px), hcols);
+        int32_t iy = fmod_i32(i32_f32_x86(py), hrows);
+        float probe = (float)hp[(size_t)iy * hcols + ix];
+        count += (((height + (t * s1)) - probe) < -0.5f) ? 1 : 0;
+    }
+    sf[(size_t)y * size + x] = __builtin_sqrtf(1.0f / ((step_size * (float)count) + 1.0f));
+}
+
 inline bool pow2(int32_t v)
 {
     return v > 0 && (v & (v - 1)) == 0;
@@ -1125,6 +1363,96 @@ int fsk_batch_run(const fsk_batch *b, hipStream_t s)
     if (b->hbits == 8)
         return b->pow2 ? launch_batch<8, true, true>(b, s) : launch_batch<8, false, false>(b, s);
     return b->pow2 ? launch_batch<16, true, true>(b, s) : launch_batch<16, false, false>(b, s);
+}
+
+int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
+                      hipStream_t s)
+{
+    if (f->rows <= 0 || f->cols <= 0)
+        return hipSuccess;
+    if (f->rows > kMaxRows)
+        return hipErrorInvalidValue;
+    RenderArgs a;
+    a.m = *m;
+    a.f = *f;
+    if (a.f.n < 0)
+        a.f.n = 0;
+    a.out = (uint32_t *)d_src;
+    a.pitch = pitch;
+    a.ntiles = 0;
+    a.chunk_max = kChunk;
+    bool hp2 = pow2(m->hrows) && pow2(m->hcols);
+    if (m->hbits == 8)
+        return hp2 ? launch_render<8, true, false, kOutSlice>(a, s)
+                   : launch_render<8, false, false, kOutSlice>(a, s);
+    return hp2 ? launch_render<16, true, false, kOutSlice>(a, s)
+               : launch_render<16, false, false, kOutSlice>(a, s);
+}
+
+static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, int32_t hc,
+                          int32_t wc, const int32_t *src)
+{
+    DiffArgs a;
+    a.f = *f;
+    a.cp = cp;
+    a.sf = sf;
+    a.hc = hc;
+    a.wc = wc;
+    a.sky_scale = 0.0f;
+    a.src = src;
+    a.gimg = nullptr;
+    a.img = nullptr;
+    a.gcp = nullptr;
+    a.gsky = nullptr;
+    return a;
+}
+
+int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                   int32_t wc, float sky_scale, const int32_t *d_src, float *d_img, hipStream_t s)
+{
+    if (f->rows <= 0 || f->cols <= 0)
+        return hipSuccess;
+    DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
+    a.sky_scale = sky_scale;
+    a.img = d_img;
+    dim3 grid((f->cols + 255) / 256, (f->rows + kDiffRows - 1) / kDiffRows);
+    hipLaunchKernelGGL(k_diff_color, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+int fsk_zero_f32(float *d_p, size_t n, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)blocks), dim3(256), 0, s, d_p, n);
+    return hipGetLastError();
+}
+
+int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                  int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
+                  float *d_gsky, hipStream_t s)
+{
+    if (f->rows <= 0 || f->cols <= 0)
+        return hipSuccess;
+    DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
+    a.gimg = d_gimg;
+    a.gcp = d_gcp;
+    a.gsky = d_gsky;
+    dim3 grid((f->cols + 255) / 256, (f->rows + kDiffRows - 1) / kDiffRows);
+    hipLaunchKernelGGL(k_diff_grad, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+int fsk_shade_factor(const uint16_t *d_hplain, int32_t hrows, int32_t hcols, float sun0,
+                     float sun1, float sun2, int32_t size, float *d_sf, hipStream_t s)
+{
+    dim3 grid((size + 255) / 256, size);
+    hipLaunchKernelGGL(k_shade_factor, grid, dim3(256), 0, s, d_hplain, hrows, hcols, sun0, sun1,
+                       sun2, size, d_sf);
+    return hipGetLastError();
 }
 
 int32_t fsk_render_max_rows(void)

@@ -1,0 +1,165 @@
+"""Differentiable renderer: torch autograd over the library's f32-plane entries.
+
+The forward pass renders a view from f32 colour planes planes[3][M][M]
+(r, g, b in [0,1]) through fsx_render_f32, which also records the depth
+slice visible at each pixel; the backward pass (fsx_render_f32_vjp)
+scatter-adds dL/dimg into dL/dplanes with the visibility held fixed.  The
+two shading parameters are (sun_height, sun_ang): sun_height scales the
+sky colour (argb.scale 0xFF9090e0 sun_height, fut/interactive.fut:163),
+so its gradient is the sky pixels' term; the shadow counts that sun_ang
+and sun_height set are piecewise constant, so their gradient through the
+shade factor is zero almost everywhere and is reported as zero.
+
+Every device call runs on the renderer context's own stream, ordered
+after torch's current stream with stream waits (no host synchronisation),
+and torch's current stream waits for it afterwards.
+"""
+import contextlib
+import ctypes
+
+import numpy as np
+import torch
+
+from .renderer import FutspaceError, View
+
+
+class DiffRenderer:
+    """The map's heights (visibility) and shade factor, resident on the
+    device; colour planes are supplied per call.
+
+    The colour planes have the shade map's size, so create the map with
+    shade_size equal to the colour map's side to differentiate with
+    respect to the full-resolution colour map."""
+
+    def __init__(self, ctx, m):
+        self.ctx = ctx
+        self.map = m
+        n = m.shade_size
+        self.hc = self.wc = n
+        self.sf = torch.empty((n, n), dtype=torch.float32, device="cuda")
+        with self.stream():
+            ctx.check(ctx.lib.fsx_map_shade_factor(ctx.ptr, m.ptr, _ptr(self.sf)),
+                      "fsx_map_shade_factor")
+
+    @contextlib.contextmanager
+    def stream(self):
+        handle = self.ctx.lib.fsx_context_stream(self.ctx.ptr)
+        if not handle:
+            raise FutspaceError("the renderer context has no stream")
+        ext = torch.cuda.ExternalStream(handle)
+        cur = torch.cuda.current_stream()
+        ext.wait_stream(cur)
+        try:
+            yield ext
+        finally:
+            cur.wait_stream(ext)
+
+    def _check_planes(self, planes):
+        if (planes.dtype != torch.float32 or not planes.is_cuda or not planes.is_contiguous()
+                or tuple(planes.shape) != (3, self.hc, self.wc)):
+            raise ValueError(f"colour planes must be a contiguous CUDA float32 tensor of shape "
+                             f"(3, {self.hc}, {self.wc}), got {tuple(planes.shape)} "
+                             f"{planes.dtype}")
+
+    def forward_raw(self, planes, sky_scale, view: View, img=None, src=None):
+        """(img[3][h][w] f32, src[h][w] i32) for one view."""
+        self._check_planes(planes)
+        if img is None:
+            img = torch.empty((3, view.h, view.w), dtype=torch.float32, device="cuda")
+        if src is None:
+            src = torch.empty((view.h, view.w), dtype=torch.int32, device="cuda")
+        cv = view.to_c()
+        with self.stream():
+            self.ctx.check(self.ctx.lib.fsx_render_f32(
+                self.ctx.ptr, self.map.ptr, ctypes.byref(cv), _ptr(planes), _ptr(self.sf),
+                self.hc, self.wc, ctypes.c_float(sky_scale), _ptr(img), _ptr(src)),
+                "fsx_render_f32")
+        return img, src
+
+    def vjp_raw(self, planes, src, gimg, view: View, gplanes=None, gsky=None, accumulate=False):
+        """dL/dplanes and dL/dsky_scale (a 1-element tensor) for one view;
+        with accumulate the gradients are added into gplanes / gsky."""
+        self._check_planes(planes)
+        gimg = gimg.contiguous()
+        if tuple(gimg.shape) != (3, view.h, view.w) or tuple(src.shape) != (view.h, view.w):
+            raise ValueError("gradient or src does not match the view")
+        if gplanes is None:
+            gplanes = torch.empty_like(planes)
+            accumulate = False
+        if gsky is None:
+            gsky = torch.empty(1, dtype=torch.float32, device="cuda")
+            accumulate = False
+        cv = view.to_c()
+        with self.stream():
+            self.ctx.check(self.ctx.lib.fsx_render_f32_vjp(
+                self.ctx.ptr, ctypes.byref(cv), _ptr(planes), _ptr(self.sf), self.hc, self.wc,
+                _ptr(src), _ptr(gimg), 1 if accumulate else 0, _ptr(gplanes), _ptr(gsky)),
+                "fsx_render_f32_vjp")
+        return gplanes, gsky
+
+    def render(self, planes, shading, view: View):
+        """Differentiable render: img[3][h][w] as a function of planes and
+        shading = (sun_height, sun_ang)."""
+        return RenderF32.apply(planes, shading, self, view)
+
+
+class RenderF32(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, planes, shading, renderer, view):
+        sky_scale = float(shading.detach().reshape(-1)[0])
+        img, src = renderer.forward_raw(planes.detach(), sky_scale, view)
+        fctx.save_for_backward(planes, src)
+        fctx.renderer = renderer
+        fctx.view = view
+        fctx.shading_meta = (shading.device, shading.dtype, tuple(shading.shape))
+        fctx.mark_non_differentiable(src)
+        return img
+
+    @staticmethod
+    def backward(fctx, gimg):
+        planes, src = fctx.saved_tensors
+        gplanes, gsky = fctx.renderer.vjp_raw(planes.detach(), src, gimg, fctx.view)
+        device, dtype, shape = fctx.shading_meta
+        gshade = torch.zeros(shape, dtype=dtype, device=device)
+        gshade.view(-1)[0] = gsky.to(device=device, dtype=dtype)[0]
+        return gplanes, gshade, None, None
+
+
+def planes_of(color):
+    """f32 colour planes [3][h][w] (r, g, b / 255) of a u32 ARGB map."""
+    c = np.asarray(color, dtype=np.uint32)
+    return np.stack([((c >> 16) & 255), ((c >> 8) & 255), (c & 255)]).astype(np.float32) / \
+        np.float32(255.0)
+
+
+def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None):
+    """Multi-view colour-map fitting: `steps` Adam steps on planes (in
+    place) against target frames targets[i] (3 x h x w f32 CUDA tensors),
+    loss = sum over views of the squared L2 distance.  Gradients of all
+    views accumulate into one buffer through the adjoint's scatter-add.
+    Returns the per-step losses (a CUDA tensor, read once at the end)."""
+    planes.requires_grad_(False)
+    opt = torch.optim.Adam([planes], lr=lr)
+    grad = torch.zeros_like(planes)
+    gsky = torch.zeros(1, dtype=torch.float32, device="cuda")
+    img = None
+    src = None
+    losses = torch.zeros(steps, dtype=torch.float32, device="cuda")
+    for it in range(steps):
+        for i, (v, t) in enumerate(zip(views, targets)):
+            if img is None or tuple(img.shape) != (3, v.h, v.w):
+                img = torch.empty((3, v.h, v.w), dtype=torch.float32, device="cuda")
+                src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
+            renderer.forward_raw(planes, sky_scale, v, img, src)
+            diff = img - t
+            losses[it] += (diff * diff).sum()
+            renderer.vjp_raw(planes, src, 2.0 * diff, v, grad, gsky, accumulate=i > 0)
+        planes.grad = grad
+        opt.step()
+        if on_step is not None:
+            on_step(it)
+    return losses
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())

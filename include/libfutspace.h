@@ -180,6 +180,37 @@ int fsx_batch_capture(struct futhark_context *ctx, struct fsx_batch *batch);
 int fsx_batch_replay(struct futhark_context *ctx, struct fsx_batch *batch);
 int fsx_batch_free(struct futhark_context *ctx, struct fsx_batch *batch);
 
+/* Differentiable renderer: the f32 relaxation of the png render path
+ * (fut/interactive.fut:179-182 over fut/voxel_renderer.fut:214-251 with
+ * png_color_filtered, fut/render_functions.fut:95-105), with the shadow
+ * counts of fut/effects.fut:108-125 and visibility held fixed.
+ *
+ * fsx_map_shade_factor writes the map's per-texel shade factor
+ * sf = sqrt(1 / (4*count + 1)) ([shade_size][shade_size] f32, device).
+ *
+ * fsx_render_f32 renders `view` from f32 colour planes
+ * dev_planes[3][hc][wc] (r, g, b in [0,1]) times dev_sf[hc][wc]:
+ * dev_img[3][h][w] f32 and dev_src[h][w], the depth slice whose sample
+ * shows at each pixel (-1: sky, whose colour is (0x90,0x90,0xE0)/255 *
+ * sky_scale).  A sample at integral x or y is transparent, as argb.mix
+ * makes it in the u32 path.  Visibility comes from the map's heights,
+ * exactly as in fsx_render.
+ *
+ * fsx_render_f32_vjp adds dL/d planes into dev_gplanes[3][hc][wc] and
+ * dL/d sky_scale into *dev_gsky, given dev_gimg = dL/d img and the src of
+ * the forward pass; accumulate = 0 clears both first.  f32 atomics: the
+ * summation order, hence the last bits, varies from run to run.
+ *
+ * All three are asynchronous on the context stream. */
+int fsx_map_shade_factor(struct futhark_context *ctx, const struct fsx_map *map, float *dev_sf);
+int fsx_render_f32(struct futhark_context *ctx, const struct fsx_map *map,
+                   const struct fsx_view *view, const float *dev_planes, const float *dev_sf,
+                   int32_t hc, int32_t wc, float sky_scale, float *dev_img, int32_t *dev_src);
+int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
+                       const float *dev_planes, const float *dev_sf, int32_t hc, int32_t wc,
+                       const int32_t *dev_src, const float *dev_gimg, int32_t accumulate,
+                       float *dev_gplanes, float *dev_gsky);
+
 /* Render kernel shape, as chosen by the autotuner: columns per workgroup
  * tile (4 or 8), workgroups per CU for 4-column tiles (1 or 2) and depth
  * slices per chunk (256-2048); 0 leaves a knob automatic.  Applies to
@@ -190,6 +221,8 @@ int fsx_set_tuning(struct futhark_context *ctx, int32_t tile_columns, int32_t wg
 /* Use `stream` (a hipStream_t, or NULL for the context's own stream) for
  * all later device work of the context. */
 int fsx_context_set_stream(struct futhark_context *ctx, void *stream);
+/* The hipStream_t the context currently enqueues on. */
+void *fsx_context_stream(struct futhark_context *ctx);
 /* HIP device index the context runs on. */
 int fsx_context_device(struct futhark_context *ctx);
 

@@ -478,12 +478,45 @@ void fso_sun(float sun_height, float sun_ang, float out[3])
 
 /* generate_shadowmap_accumulated, fut/effects.fut:108-125, with the
  * output side `size` (1024 in the reference). */
+#define SHADOW_MAX_DIST 1024
+#define SHADOW_STEPS 256
+
+/* Sun-ray steps from texel (x, y) that pass more than half a unit below
+ * the terrain, fut/effects.fut:116-122. */
+static int32_t shadow_count(const int32_t *alt, int32_t ha, int32_t wa, const float sun[3],
+                            int32_t x, int32_t y)
+{
+    const float step_size = (float)(SHADOW_MAX_DIST / SHADOW_STEPS);
+    float fx = (float)x, fy = (float)y;
+    float height = fso_png_height(alt, ha, wa, fx, fy);
+    int32_t count = 0;
+    for (int32_t dist = 1; dist < SHADOW_STEPS; dist++) {
+        float fdist = (float)dist;
+        float t = fdist * step_size;
+        float probe = fso_png_height(alt, ha, wa, fx + (t * sun[0]), fy + (t * sun[2]));
+        if (((height + (t * sun[1])) - probe) < -0.5f)
+            count += 1;
+    }
+    return count;
+}
+
+int fso_shadow_counts(const int32_t *alt, int32_t ha, int32_t wa, const float sun[3],
+                      int32_t size, int32_t *counts)
+{
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int32_t y = 0; y < size; y++)
+        for (int32_t x = 0; x < size; x++)
+            counts[(size_t)y * size + x] = shadow_count(alt, ha, wa, sun, x, y);
+    return 0;
+}
+
 int fso_shadowmap(const uint32_t *col, int32_t hc, int32_t wc,
                   const int32_t *alt, int32_t ha, int32_t wa,
                   const float sun[3], int32_t size, uint32_t *out)
 {
-    const int32_t max_dist = 1024, steps_per_ray = 256;
-    const float step_size = (float)(max_dist / steps_per_ray);
+    const float step_size = (float)(SHADOW_MAX_DIST / SHADOW_STEPS);
     const uint32_t black = fso_black();
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
@@ -491,16 +524,7 @@ int fso_shadowmap(const uint32_t *col, int32_t hc, int32_t wc,
     for (int32_t y = 0; y < size; y++) {
         for (int32_t x = 0; x < size; x++) {
             float fx = (float)x, fy = (float)y;
-            float height = fso_png_height(alt, ha, wa, fx, fy);
-            int32_t count = 0;
-            for (int32_t dist = 1; dist < steps_per_ray; dist++) {
-                float fdist = (float)dist;
-                float t = fdist * step_size;
-                float probe = fso_png_height(alt, ha, wa, fx + (t * sun[0]), fy + (t * sun[2]));
-                if (((height + (t * sun[1])) - probe) < -0.5f)
-                    count += 1;
-            }
-            float amount = (float)count;
+            float amount = (float)shadow_count(alt, ha, wa, sun, x, y);
             out[(size_t)y * size + x] =
                 fso_mix(step_size * amount, black, 1.0f, fso_png_color(col, hc, wc, fx, fy));
         }

@@ -118,6 +118,10 @@ void fso_sun(float sun_height, float sun_ang, float out[3]);
 int  fso_shadowmap(const uint32_t *col, int32_t hc, int32_t wc,
                    const int32_t *alt, int32_t ha, int32_t wa,
                    const float sun[3], int32_t size, uint32_t *out);
+/* The per-texel count of shadowing sun-ray steps that fso_shadowmap mixes
+ * with black (fut/effects.fut:116-122). */
+int  fso_shadow_counts(const int32_t *alt, int32_t ha, int32_t wa, const float sun[3],
+                       int32_t size, int32_t *counts);
 uint32_t fso_sky(float sun_height);
 
 /* ---- stateless benchmark program (tests/futspace.fut:47-145) ---- */

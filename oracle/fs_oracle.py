@@ -56,6 +56,7 @@ _SIG = {
     "fso_vec3_rotate": (None, [ctypes.c_int, F, P, P]),
     "fso_sun": (None, [F, F, P]),
     "fso_shadowmap": (ctypes.c_int, [P, I, I, P, I, I, P, I, P]),
+    "fso_shadow_counts": (ctypes.c_int, [P, I, I, P, I, P]),
     "fso_sky": (U, [F]),
     "fso_render_nearest": (ctypes.c_int, [CP, P, P, I, I, I, I, I, P]),
     "fso_render_nearest_painter": (ctypes.c_int, [CP, P, P, I, I, I, I, I, P]),
@@ -161,6 +162,23 @@ def shadowmap(color, alt, sun_vec, size=1024):
                              a.shape[1], s.ctypes.data, size, out.ctypes.data)
     assert rc == 0
     return out
+
+
+def shadow_counts(alt, sun_vec, size=1024):
+    """Per-texel shadowing step count of the shading pass (fut/effects.fut:116-122)."""
+    a, s = _i32(alt), _f32(sun_vec)
+    out = np.empty((size, size), dtype=np.int32)
+    rc = lib().fso_shadow_counts(a.ctypes.data, a.shape[0], a.shape[1], s.ctypes.data, size,
+                                 out.ctypes.data)
+    assert rc == 0
+    return out
+
+
+def shade_factor(counts):
+    """The relaxation's per-texel shade factor sqrt(1 / (4*count + 1)), in f32."""
+    f = np.float32
+    return np.sqrt(f(1.0) / (f(4.0) * np.asarray(counts).astype(np.float32) + f(1.0))).astype(
+        np.float32)
 
 
 def update_map(color, height, sun_height=0.1, sun_ang=0.1, size=1024, bits=8):

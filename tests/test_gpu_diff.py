@@ -1,0 +1,194 @@
+"""Differentiable renderer (needs a GPU).
+
+The HIP forward must reproduce the oracle's f32 relaxation (visible slice
+per pixel exactly, colours bit for bit), the adjoint must match the
+oracle's vjp within 1e-5 of the gradient's scale, and both must agree
+with central finite differences of the loss.
+"""
+import numpy as np
+import pytest
+
+from futspace_amd import View, scene, sky_color
+from oracle import fs_oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from futspace_amd.diff import DiffRenderer, fit, planes_of  # noqa: E402
+
+SUN_H, SUN_A = 0.1, 0.1
+
+
+def _setup(ctx, seed=11, size=128, bits=8, sun=(SUN_H, SUN_A)):
+    if bits == 8:
+        col, hts = scene.scene_maps(seed, size)
+    else:
+        hts = scene.heightmap(seed, size, 0.6, 600)
+        col = scene.colormap(seed, hts, 600)
+    m = ctx.map(col, hts, height_bits=bits, shade_size=size, sun_height=sun[0], sun_ang=sun[1])
+    r = DiffRenderer(ctx, m)
+    alt = (np.asarray(hts, dtype=np.int32) & ((1 << bits) - 1)).astype(np.int32)
+    counts = O.shadow_counts(alt, O.sun(*sun), size)
+    sf = O.shade_factor(counts)
+    return r, col, alt, sf
+
+
+VIEWS = [
+    View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=60.0, distance=300.0, fov=1.2,
+         h=96, w=128, d=0.01, z0=0.5),
+    View(x=64.0, y=64.0, height=200.0, angle=0.0, horizon=50.0, distance=200.0, fov=1.2,
+         h=64, w=80, d=0.01, z0=0.5),                       # yaw 0: integral x, transparent
+    View(x=10.5, y=3.25, height=300.0, angle=1.5707964, horizon=90.0, distance=400.0, fov=0.8,
+         h=100, w=77, d=0.005, z0=1.0, scale=120.0),
+    View(x=33.3, y=77.7, height=250.0, angle=5.0, horizon=300.0, distance=700.0, fov=1.2,
+         h=257, w=129, d=0.003, z0=0.4, scale=64.0, n_slices=3000),
+]
+
+
+def test_shade_factor_matches_oracle(ctx):
+    for sun in [(SUN_H, SUN_A), (0.6, 2.0), (-0.3, 0.7)]:
+        r, _, _, sf = _setup(ctx, sun=sun)
+        torch.cuda.synchronize()
+        assert np.array_equal(r.sf.cpu().numpy().view(np.uint32), sf.view(np.uint32)), sun
+
+
+@pytest.mark.parametrize("vi", range(len(VIEWS)))
+def test_forward_matches_oracle(ctx, vi):
+    r, col, alt, sf = _setup(ctx)
+    v = VIEWS[vi]
+    planes = planes_of(col)
+    img, src = r.forward_raw(torch.from_numpy(planes).cuda(), 0.37, v)
+    torch.cuda.synchronize()
+    want_img, want_src = O.render_f32(v, planes, sf, alt, 0.37)
+    got_src = src.cpu().numpy()
+    cols = np.arange(v.w, dtype=np.int32)[None, :]
+    got_src_o = np.where(got_src < 0, -1, got_src * v.w + cols)
+    assert np.array_equal(got_src_o, want_src), f"{np.count_nonzero(got_src_o != want_src)} pixels"
+    got = img.cpu().numpy()
+    bad = np.count_nonzero(got.view(np.uint32) != want_img.view(np.uint32))
+    assert bad == 0, f"{bad} channel values differ, max abs {np.abs(got - want_img).max()}"
+
+
+def test_forward_sixteen_bit_heights(ctx):
+    r, col, alt, sf = _setup(ctx, seed=13, bits=16)
+    v = View(x=50.0, y=90.0, height=700.0, angle=1.0, horizon=120.0, distance=1000.0, fov=1.2,
+             h=144, w=160, d=0.002, z0=0.5)
+    planes = planes_of(col)
+    img, src = r.forward_raw(torch.from_numpy(planes).cuda(), 0.1, v)
+    want_img, want_src = O.render_f32(v, planes, sf, alt, 0.1)
+    got_src = src.cpu().numpy()
+    cols = np.arange(v.w, dtype=np.int32)[None, :]
+    assert np.array_equal(np.where(got_src < 0, -1, got_src * v.w + cols), want_src)
+    assert np.array_equal(img.cpu().numpy(), want_img)
+
+
+def _rel_err(got, want):
+    scale = max(float(np.abs(want).max()), 1e-30)
+    return float(np.abs(got.astype(np.float64) - want.astype(np.float64)).max()) / scale
+
+
+@pytest.mark.parametrize("vi", range(len(VIEWS)))
+def test_vjp_matches_oracle(ctx, vi):
+    """Colour-plane gradients within 1e-5 of the largest gradient; the sky
+    gradient within 1e-5 of the L1 mass of its terms (f32 atomics sum it
+    in no fixed order)."""
+    r, col, alt, sf = _setup(ctx)
+    v = VIEWS[vi]
+    planes = planes_of(col)
+    rng = np.random.default_rng(100 + vi)
+    gimg = rng.standard_normal((3, v.h, v.w)).astype(np.float32)
+    tp = torch.from_numpy(planes).cuda()
+    _, src = r.forward_raw(tp, 0.37, v)
+    gp, gs = r.vjp_raw(tp, src, torch.from_numpy(gimg).cuda(), v)
+    torch.cuda.synchronize()
+    _, want_src = O.render_f32(v, planes, sf, alt, 0.37)
+    want_gp, want_gs = O.render_f32_vjp(v, planes, sf, want_src, gimg)
+    assert _rel_err(gp.cpu().numpy(), want_gp) <= 1e-5
+    sky_terms = np.abs(gimg[:, want_src < 0]).sum(axis=1) @ np.array([144, 144, 224]) / 255.0
+    assert abs(float(gs.cpu()[0]) - want_gs) <= 1e-5 * max(sky_terms, 1e-3)
+
+
+def test_vjp_accumulates(ctx):
+    r, col, _, _ = _setup(ctx)
+    tp = torch.from_numpy(planes_of(col)).cuda()
+    g = torch.zeros_like(tp)
+    gs = torch.zeros(1, device="cuda")
+    total = torch.zeros_like(tp)
+    for i, v in enumerate(VIEWS[:3]):
+        _, src = r.forward_raw(tp, 0.2, v)
+        gimg = torch.ones((3, v.h, v.w), device="cuda")
+        one, _ = r.vjp_raw(tp, src, gimg, v)
+        total += one
+        r.vjp_raw(tp, src, gimg, v, g, gs, accumulate=i > 0)
+    torch.cuda.synchronize()
+    assert torch.allclose(g, total, rtol=1e-5, atol=1e-6)
+
+
+def _loss64(r, planes, v, target, sky):
+    img, _ = r.forward_raw(planes, sky, v)
+    d = (img - target).double()
+    return float((d * d).sum())
+
+
+def test_autograd_function_and_finite_differences(ctx):
+    """dL/dplanes through torch.autograd equals the adjoint, and agrees with
+    central differences (h = 1e-2) on 64 texels, mostly visible ones."""
+    r, col, alt, sf = _setup(ctx, seed=21, size=256)
+    v = View(x=128.3, y=40.7, height=230.0, angle=2.7, horizon=110.0, distance=500.0, fov=1.2,
+             h=180, w=240, d=0.002, z0=0.5, n_slices=1200)
+    rng = np.random.default_rng(9)
+    planes0 = planes_of(col)
+    target = np.clip(planes0 + rng.normal(0.0, 8.0 / 255.0, planes0.shape), 0.0, 1.0)
+    tgt_img, _ = r.forward_raw(torch.from_numpy(target.astype(np.float32)).cuda(), SUN_H, v)
+    planes = torch.from_numpy(planes0).cuda().requires_grad_(True)
+    shading = torch.tensor([0.15, SUN_A], dtype=torch.float32, requires_grad=True)
+    img = r.render(planes, shading, v)
+    loss = ((img - tgt_img) ** 2).sum()
+    loss.backward()
+    g = planes.grad.detach().clone()
+    # the adjoint called directly
+    _, src = r.forward_raw(planes.detach(), 0.15, v)
+    g2, gs2 = r.vjp_raw(planes.detach(), src, 2.0 * (img.detach() - tgt_img), v)
+    assert float((g - g2).abs().max()) <= 1e-5 * float(g2.abs().max())
+    assert abs(float(shading.grad[0]) - float(gs2.cpu()[0])) <= 1e-4 * abs(float(gs2.cpu()[0]))
+    assert float(shading.grad[1]) == 0.0
+    # central differences
+    # texels whose value stays positive under +-h (the channel mix is even
+    # in each texel value, so FD across 0 would not be a derivative)
+    h = 1e-2
+    gh = g.cpu().numpy()
+    ok = planes0 > 5 * h
+    nz = np.argwhere((np.abs(gh) > 1e-3 * np.abs(gh).max()) & ok)
+    pick = nz[rng.choice(len(nz), 48, replace=False)]
+    allok = np.argwhere(ok)
+    rand = allok[rng.choice(len(allok), 16, replace=False)]
+    base = planes.detach().clone()
+    for ch, y, x in list(map(tuple, pick)) + list(map(tuple, rand)):
+        p = base.clone()
+        p[ch, y, x] += h
+        lp = _loss64(r, p, v, tgt_img, 0.15)
+        p[ch, y, x] -= 2 * h
+        lm = _loss64(r, p, v, tgt_img, 0.15)
+        fd = (lp - lm) / (2 * h)
+        an = float(gh[ch, y, x])
+        assert abs(fd - an) <= 2e-2 * max(abs(an), abs(fd)) + 1e-3, (ch, y, x, fd, an)
+    # sun_height through the sky pixels
+    def loss_sky(s):
+        return _loss64(r, base, v, tgt_img, s)
+    fd = (loss_sky(0.15 + h) - loss_sky(0.15 - h)) / (2 * h)
+    an = float(shading.grad[0])
+    assert abs(fd - an) <= 1e-3 * max(abs(an), 1.0), (fd, an)
+
+
+def test_fit_reduces_loss(ctx):
+    r, col, _, _ = _setup(ctx, seed=31, size=128)
+    rng = np.random.default_rng(8)
+    planes0 = planes_of(col)
+    target_planes = np.clip(planes0 + rng.normal(0.0, 8.0 / 255.0, planes0.shape), 0, 1)
+    views = VIEWS[:3]
+    tp = torch.from_numpy(target_planes.astype(np.float32)).cuda()
+    targets = [r.forward_raw(tp, SUN_H, v)[0].clone() for v in views]
+    planes = torch.from_numpy(planes0).cuda()
+    losses = fit(r, planes, views, targets, SUN_H, steps=20, lr=0.01).cpu().numpy()
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.5 * losses[0], losses
