@@ -33,6 +33,10 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-view leg")
     ap.add_argument("--batch-reps", type=int, default=20,
                     help="timed graph replays of the 256-view batch")
+    ap.add_argument("--no-diff", action="store_true",
+                    help="skip the differentiation legs (2048^2 grad step, 8192^2 fit)")
+    ap.add_argument("--no-fit", action="store_true", help="skip the 8192^2 multi-view fit")
+    ap.add_argument("--fit-steps", type=int, default=50)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 64 MiB device copy (known bytes) to calibrate HBM counters")
     return ap.parse_args()
@@ -115,6 +119,114 @@ def bench_batched(ctx, reps):
     }
 
 
+def _kernel_ms(ctx, fn, reps):
+    ctx.sync()
+    ctx.profile(True)
+    for _ in range(reps):
+        fn()
+    ms, n = ctx.profile_read()
+    ctx.profile(False)
+    return ms / max(n, 1)
+
+
+def bench_diff(ctx, reps):
+    """One 1280x720 view of the 2048^2 map, 2048 slices: forward (f32
+    planes) and forward + L2 loss + adjoint, the gradient w.r.t. the
+    [3, 2048, 2048] colour planes and the shading parameters."""
+    import torch
+
+    from futspace_amd import scene
+    from futspace_amd.diff import DiffRenderer, planes_of
+
+    col, hts = scene.diff_2048_maps()
+    m = ctx.map(col, hts, shade_size=2048)
+    r = DiffRenderer(ctx, m)
+    v = scene.diff_2048_view(hts)
+    planes_np = planes_of(col)
+    planes = torch.from_numpy(planes_np).cuda()
+    target_planes = torch.from_numpy(
+        scene.perturbed_planes(planes_np, scene.DIFF_2048_TARGET_SEED)).cuda()
+    target, _ = r.forward_raw(target_planes, scene.SUN_HEIGHT, v)
+    target = target.clone()
+    img = torch.empty((3, v.h, v.w), dtype=torch.float32, device="cuda")
+    src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
+    gplanes = torch.empty_like(planes)
+    gsky = torch.empty(1, dtype=torch.float32, device="cuda")
+
+    def fwd():
+        r.forward_raw(planes, scene.SUN_HEIGHT, v, img, src)
+
+    def step():
+        r.forward_raw(planes, scene.SUN_HEIGHT, v, img, src)
+        diff = img - target
+        loss = (diff * diff).sum()
+        r.vjp_raw(planes, src, 2.0 * diff, v, gplanes, gsky)
+        return loss
+
+    for _ in range(5):
+        step()
+    torch.cuda.synchronize()
+    fwd_kernel_ms = _kernel_ms(ctx, fwd, reps)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd()
+    torch.cuda.synchronize()
+    fwd_ms = 1e3 * (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss = step()
+    torch.cuda.synchronize()
+    step_ms = 1e3 * (time.perf_counter() - t0) / reps
+    # the adjoint alone (clear + scatter-add), kernel events
+    diff = 2.0 * (img - target)
+    vjp_kernel_ms = _kernel_ms(ctx, lambda: r.vjp_raw(planes, src, diff, v, gplanes, gsky), reps)
+    visible = int((src >= 0).sum())
+    return {"workload": "1280x720 view, 2048^2 map, 2048 slices, grad w.r.t. f32 "
+                        "[3,2048,2048] colour planes + (sun_height, sun_ang)",
+            "fwd_ms": fwd_ms, "fwd_bwd_step_ms": step_ms, "step_over_fwd": step_ms / fwd_ms,
+            "fwd_kernel_ms": fwd_kernel_ms, "vjp_kernel_ms": vjp_kernel_ms,
+            "loss": float(loss), "terrain_pixels": visible}
+
+
+def bench_fit(ctx, steps):
+    """64 views of 3840x2160 with 4096 slices over the 8192^2 map with
+    16-bit heights: colour-map fitting with Adam (lr 0.5), the views'
+    gradients accumulated by the adjoint's scatter-add."""
+    import numpy as np
+    import torch
+
+    from futspace_amd import scene
+    from futspace_amd.diff import DiffRenderer, fit, planes_of
+
+    t0 = time.perf_counter()
+    col, hts = scene.diff_8192_maps()
+    m = ctx.map(col, hts, height_bits=16, shade_size=8192)
+    r = DiffRenderer(ctx, m)
+    views = scene.diff_8192_views(hts)
+    planes_np = planes_of(col)
+    del col
+    target_planes = torch.from_numpy(
+        scene.perturbed_planes(planes_np, scene.DIFF_8192_SEED)).cuda()
+    targets = [r.forward_raw(target_planes, scene.SUN_HEIGHT, v)[0] for v in views]
+    del target_planes
+    planes = torch.from_numpy(planes_np).cuda()
+    del planes_np
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    fit(r, planes.clone(), views[:2], targets[:2], scene.SUN_HEIGHT, steps=1, lr=0.5)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = fit(r, planes, views, targets, scene.SUN_HEIGHT, steps=steps, lr=0.5)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ls = losses.cpu().numpy()
+    return {"workload": "64 views 3840x2160, 4096 slices, 8192^2 u16 heights + colour, "
+                        f"{steps} Adam steps lr 0.5",
+            "ms_per_step": 1e3 * dt / steps, "loss_first": float(ls[0]),
+            "loss_last": float(ls[-1]), "loss_every_10": [float(x) for x in ls[::10]],
+            "setup_s": setup_s}
+
+
 def cpu_baseline(views, shaded, alt, budget_s):
     """The oracle painter (the futhark multicore stand-in) on this host's
     cores over a bounded sample of the same fly-through."""
@@ -122,7 +234,7 @@ def cpu_baseline(views, shaded, alt, budget_s):
     threads = len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     frames = 0
-    while frames < len(views):
+    while True:
         O.render_painter_mt(views[frames % len(views)], shaded, alt, threads)
         frames += 1
         if time.perf_counter() - t0 > budget_s:
@@ -142,8 +254,8 @@ def cpu_baseline(views, shaded, alt, budget_s):
     except OSError:
         pass
     return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} frames of the 1920x1080 / 2048-slice fly-through "
-                      f"({dt:.1f} s), oracle painter, OpenMP over columns",
+            "sample": f"{frames} frames cycling through the 1920x1080 / 2048-slice "
+                      f"fly-through ({dt:.1f} s), oracle painter, OpenMP over columns",
             "single_thread_frames_per_s": 1.0 / st, "cpu_model": cpu_model}
 
 
@@ -219,6 +331,11 @@ def main():
     if not args.no_batch:
         ctx.set_stream(None)
         rec["batched"] = bench_batched(ctx, max(3, args.batch_reps if not args.quick else 3))
+    if not args.no_diff:
+        ctx.set_stream(None)
+        rec["diff"] = bench_diff(ctx, 50 if not args.quick else 5)
+        if not args.no_fit:
+            rec["fit"] = bench_fit(ctx, args.fit_steps if not args.quick else 3)
     if args.calib_copy:
         calib_copy()
     if not args.no_cpu:

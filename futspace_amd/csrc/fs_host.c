@@ -1361,8 +1361,8 @@ static int diff_frame(struct futhark_context *ctx, const char *what, const struc
         set_error(ctx, "%s: unsupported frame size %d x %d", what, v->h, v->w);
         return 1;
     }
-    if (hc <= 0 || wc <= 0) {
-        set_error(ctx, "%s: colour planes of %d x %d", what, hc, wc);
+    if (hc <= 0 || wc <= 0 || (int64_t)hc * wc > ((int64_t)1 << 31)) {
+        set_error(ctx, "%s: colour planes of %d x %d (at most 2^31 texels)", what, hc, wc);
         return 1;
     }
     frame_of_view(v, f);

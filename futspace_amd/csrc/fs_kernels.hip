@@ -1123,7 +1123,6 @@ hipError_t launch_batch(const fsk_batch *fb, hipStream_t s)
  * are sky_base * sky_scale with sky_base = (0x90, 0x90, 0xE0) / 255
  * (argb.scale of 0xFF9090e0, fut/interactive.fut:163). */
 
-constexpr int kDiffRows = 16;          /* rows one thread walks down a column */
 
 struct DiffArgs {
     fsk_frame f;
@@ -1180,109 +1179,172 @@ __device__ __forceinline__ float sky_base(int ch)
     return ch == 2 ? 224.0f / 255.0f : 144.0f / 255.0f;
 }
 
-/* Forward colour: each thread walks kDiffRows rows of one column and
- * evaluates a sample once per run of equal src. */
-__global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
+/* Colour of one visible sample, per channel (the relaxed png_color_filtered). */
+__device__ __forceinline__ void sample_colour(const DiffArgs &a, const Bilin &b, size_t plane,
+                                              float sv[3][4], float px[3])
 {
-    const int32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t r0 = blockIdx.y * kDiffRows;
-    const int32_t H = a.f.rows, W = a.f.cols;
-    if (col >= W)
-        return;
-    const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
-    int32_t last = -2;
-    float px[3] = {0.0f, 0.0f, 0.0f};
-    for (int32_t r = r0; r < min(r0 + kDiffRows, H); r++) {
-        const size_t p = (size_t)r * W + col;
-        const int32_t k = a.src[p];
-        if (k != last) {
-            last = k;
-            if (k < 0) {
+    float sf[4];
 #pragma unroll
-                for (int ch = 0; ch < 3; ch++)
-                    px[ch] = sky_base(ch) * a.sky_scale;
-            } else {
-                Bilin b = bilin_of(a.f, k, col, a.hc, a.wc);
-                float sf[4];
+    for (int t = 0; t < 4; t++)
+        sf[t] = a.sf[b.t[t]];
 #pragma unroll
-                for (int t = 0; t < 4; t++)
-                    sf[t] = a.sf[b.t[t]];
+    for (int ch = 0; ch < 3; ch++) {
+        const float *P = a.cp + ch * plane;
 #pragma unroll
-                for (int ch = 0; ch < 3; ch++) {
-                    const float *P = a.cp + ch * plane;
-                    float s0 = P[b.t[0]] * sf[0], s1 = P[b.t[1]] * sf[1];
-                    float s2 = P[b.t[2]] * sf[2], s3 = P[b.t[3]] * sf[3];
-                    float X1 = mixf(b.wx0, s0, b.wx1, s1);
-                    float X2 = mixf(b.wx0, s2, b.wx1, s3);
-                    px[ch] = mixf(b.wy0, X1, b.wy1, X2);
-                }
-            }
-        }
-#pragma unroll
-        for (int ch = 0; ch < 3; ch++)
-            a.img[ch * fplane + p] = px[ch];
+        for (int t = 0; t < 4; t++)
+            sv[ch][t] = P[b.t[t]] * sf[t];
+        float X1 = mixf(b.wx0, sv[ch][0], b.wx1, sv[ch][1]);
+        float X2 = mixf(b.wx0, sv[ch][2], b.wx1, sv[ch][3]);
+        px[ch] = mixf(b.wy0, X1, b.wy1, X2);
     }
 }
 
-/* Adjoint: per run of equal src the incoming gradient is summed, then
- * pushed through the three mixes once.  For px = mix(wy0, X1, wy1, X2),
- * X1 = mix(wx0, s0, wx1, s1): d px / d s0 = wy0' * wx0' * s0 / px (X1
- * cancels), and likewise for the other texels; 0 where px == 0.  The
- * texel gradients (times sf, d s / d cp) are scatter-added with f32
- * atomics; the sky term is reduced per wave. */
+/* Forward colour, one thread per pixel (2-D blocks of 64 columns x 4
+ * rows: the gathers of neighbouring pixels share texels). */
+__global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
+{
+    const int32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int32_t row = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int32_t H = a.f.rows, W = a.f.cols;
+    if (col >= W || row >= H)
+        return;
+    const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
+    const size_t p = (size_t)row * W + col;
+    const int32_t k = a.src[p];
+    float px[3];
+    if (k < 0) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++)
+            px[ch] = sky_base(ch) * a.sky_scale;
+    } else {
+        float sv[3][4];
+        sample_colour(a, bilin_of(a.f, k, col, a.hc, a.wc), plane, sv, px);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++)
+        a.img[ch * fplane + p] = px[ch];
+}
+
+/* Segmented suffix sum over the lanes of a wave: lanes holding the same
+ * key in a contiguous run form a segment, and each segment's first lane
+ * ends up with the segment's total of v[0..2].  Returns whether this
+ * lane heads its segment. */
+__device__ __forceinline__ bool segment_sum3(uint32_t key, float v[3])
+{
+    const int32_t lane = threadIdx.x & 63;
+    uint32_t next = __shfl_down(key, 1);
+    uint32_t prev = __shfl_up(key, 1);
+    int32_t tail = (lane == 63 || next != key) ? 1 : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        float o0 = __shfl_down(v[0], off), o1 = __shfl_down(v[1], off);
+        float o2 = __shfl_down(v[2], off);
+        int32_t to = __shfl_down(tail, off);
+        if (lane + off > 63) {
+            o0 = o1 = o2 = 0.0f;
+            to = 1;
+        }
+        if (!tail) {
+            v[0] += o0;
+            v[1] += o1;
+            v[2] += o2;
+        }
+        tail |= to;
+    }
+    return lane == 0 || prev != key;
+}
+
+constexpr int kGradRows = 8;           /* rows per thread in the adjoint */
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+/* Adjoint.  Each thread holds kGradRows rows of one column; a wave is 64
+ * adjacent columns of the same rows.  A run of equal src down the column
+ * is one visible sample: its incoming gradient is summed in registers and
+ * pushed through the three mixes once, when the run ends.  For
+ * px = mix(wy0, X1, wy1, X2), X1 = mix(wx0, s0, wx1, s1):
+ * d px / d s0 = wy0' * wx0' * s0 / px (X1 cancels), likewise for the other
+ * texels, 0 where px == 0.  Before the f32 atomics, lanes of the wave that
+ * end a run on the same row and hit the same texel (adjacent columns
+ * magnify the same near texel) are summed by a segmented reduction, so a
+ * hot texel receives one atomic per wave instead of one per column.  The
+ * sky term is reduced per wave. */
 __global__ __launch_bounds__(256) void k_diff_grad(DiffArgs a)
 {
-    const int32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t r0 = blockIdx.y * kDiffRows;
+    const int32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int32_t r0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * kGradRows;
     const int32_t H = a.f.rows, W = a.f.cols;
+    const bool ok = col < W && r0 < H;
+    const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
+    int32_t src[kGradRows];
+    float g[3][kGradRows];
+#pragma unroll
+    for (int j = 0; j < kGradRows; j++) {
+        const int32_t r = r0 + j;
+        const bool in = ok && r < H;
+        const size_t p = (size_t)r * W + col;
+        src[j] = in ? a.src[p] : INT32_MIN;
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++)
+            g[ch][j] = in ? a.gimg[ch * fplane + p] : 0.0f;
+    }
     float sky_acc = 0.0f;
-    if (col < W) {
-        const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
-        const int32_t r1 = min(r0 + kDiffRows, H);
-        int32_t r = r0;
-        while (r < r1) {
-            const int32_t k = a.src[(size_t)r * W + col];
-            float g[3] = {0.0f, 0.0f, 0.0f};
-            int32_t e = r;
-            for (; e < r1 && a.src[(size_t)e * W + col] == k; e++) {
+    float run[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int ch = 0; ch < 3; ch++)
-                    g[ch] += a.gimg[ch * fplane + (size_t)e * W + col];
-            }
-            r = e;
-            if (k < 0) {
+    for (int j = 0; j < kGradRows; j++) {
 #pragma unroll
-                for (int ch = 0; ch < 3; ch++)
-                    sky_acc += g[ch] * sky_base(ch);
-                continue;
-            }
-            Bilin b = bilin_of(a.f, k, col, a.hc, a.wc);
-            float sf[4];
+        for (int ch = 0; ch < 3; ch++)
+            run[ch] += g[ch][j];
+        const int32_t k = src[j];
+        const bool ends = (j == kGradRows - 1 || src[j + 1] != k) && k != INT32_MIN;
+        if (ends && k < 0) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++)
+                sky_acc += run[ch] * sky_base(ch);
+        }
+        const bool flush = ends && k >= 0;
+        if (__ballot(flush)) {
+            float val[4][3];
+            uint32_t key[4] = {kNoKey, kNoKey, kNoKey, kNoKey};
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                sf[t] = a.sf[b.t[t]];
-            const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
-            const float wx0n = b.wx0 / wxs, wx1n = b.wx1 / wxs;
-            const float wy0n = b.wy0 / wys, wy1n = b.wy1 / wys;
-            const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
+                val[t][0] = val[t][1] = val[t][2] = 0.0f;
+            if (flush) {
+                Bilin b = bilin_of(a.f, k, col, a.hc, a.wc);
+                float sv[3][4], px[3];
+                sample_colour(a, b, plane, sv, px);
+                const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
+                const float wx0n = b.wx0 / wxs, wx1n = b.wx1 / wxs;
+                const float wy0n = b.wy0 / wys, wy1n = b.wy1 / wys;
+                const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
+                float sf[4];
 #pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                const float *P = a.cp + ch * plane;
-                float s[4];
+                for (int t = 0; t < 4; t++) {
+                    sf[t] = a.sf[b.t[t]];
+                    key[t] = (uint32_t)b.t[t];
+                }
 #pragma unroll
-                for (int t = 0; t < 4; t++)
-                    s[t] = P[b.t[t]] * sf[t];
-                float X1 = mixf(b.wx0, s[0], b.wx1, s[1]);
-                float X2 = mixf(b.wx0, s[2], b.wx1, s[3]);
-                float px = mixf(b.wy0, X1, b.wy1, X2);
-                if (!(px > 0.0f))
-                    continue;
-                float q = g[ch] / px;
-                float *G = a.gcp + ch * plane;
+                for (int ch = 0; ch < 3; ch++) {
+                    if (!(px[ch] > 0.0f))
+                        continue;
+                    const float q = run[ch] / px[ch];
 #pragma unroll
-                for (int t = 0; t < 4; t++)
-                    unsafeAtomicAdd(&G[b.t[t]], ((q * wt[t]) * s[t]) * sf[t]);
+                    for (int t = 0; t < 4; t++)
+                        val[t][ch] = ((q * wt[t]) * sv[ch][t]) * sf[t];
+                }
             }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const bool head = segment_sum3(key[t], val[t]);
+                if (head && key[t] != kNoKey) {
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++)
+                        if (val[t][ch] != 0.0f)
+                            unsafeAtomicAdd(&a.gcp[ch * plane + key[t]], val[t][ch]);
+                }
+            }
+        }
+        if (ends) {
+            run[0] = run[1] = run[2] = 0.0f;
         }
     }
 #pragma unroll
@@ -1415,7 +1477,7 @@ int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int
     DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
     a.sky_scale = sky_scale;
     a.img = d_img;
-    dim3 grid((f->cols + 255) / 256, (f->rows + kDiffRows - 1) / kDiffRows);
+    dim3 grid((f->cols + 63) / 64, (f->rows + 3) / 4);
     hipLaunchKernelGGL(k_diff_color, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
@@ -1441,7 +1503,7 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
     a.gimg = d_gimg;
     a.gcp = d_gcp;
     a.gsky = d_gsky;
-    dim3 grid((f->cols + 255) / 256, (f->rows + kDiffRows - 1) / kDiffRows);
+    dim3 grid((f->cols + 63) / 64, (f->rows + 4 * kGradRows - 1) / (4 * kGradRows));
     hipLaunchKernelGGL(k_diff_grad, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -176,11 +176,37 @@ def batch_offsets(views):
     return np.asarray(offs, dtype=np.int64), total
 
 
-def diff_2048_view():
+DIFF_2048_TARGET_SEED = 8
+DIFF_2048_TEXEL_SEED = 9
+DIFF_8192_SEED = 10
+DIFF_8192_MAX_HEIGHT = 1024
+
+
+def diff_2048_maps():
+    """The differentiation workload's 2048^2 map: the plain 2048^2 pair."""
+    return plain_2048_maps()
+
+
+def diff_2048_view(heights):
     d, z0 = depth_for(2048)
-    return View(x=1024.0, y=1024.0, height=250.0, angle=0.7, horizon=150.0, distance=1000.0,
-                fov=1.2, sky_color=sky_color(SUN_HEIGHT), h=720, w=1280, d=d, z0=z0,
-                scale=0.0, n_slices=2048)
+    alt = above_ground(250.0, heights, 1024.0, 1024.0)
+    return View(x=1024.0, y=1024.0, height=float(np.float32(alt)), angle=0.7, horizon=150.0,
+                distance=1000.0, fov=1.2, sky_color=sky_color(SUN_HEIGHT), h=720, w=1280, d=d,
+                z0=z0, scale=0.0, n_slices=2048)
+
+
+def diff_8192_maps():
+    """8192^2 16-bit heights (0..1024) and their colour map, seed 10."""
+    hts = heightmap(DIFF_8192_SEED, 8192, 0.6, DIFF_8192_MAX_HEIGHT)
+    return colormap(DIFF_8192_SEED, hts, DIFF_8192_MAX_HEIGHT, 12), hts
+
+
+def perturbed_planes(planes, seed, sigma=8.0):
+    """Colour planes plus seeded N(0, sigma^2) noise per channel (in byte
+    units), clipped to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    noisy = planes + rng.normal(0.0, sigma / 255.0, planes.shape).astype(np.float32)
+    return np.clip(noisy, 0.0, 1.0).astype(np.float32)
 
 
 def diff_8192_views(heights, n_views=64, seed=10):
