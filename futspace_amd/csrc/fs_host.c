@@ -1429,6 +1429,23 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
     return 0;
 }
 
+int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *dev_target,
+                int64_t n, int32_t accumulate, float *dev_grad, float *dev_loss)
+{
+    if (!ctx)
+        return 1;
+    if (n < 0 || ((!dev_img || !dev_target || !dev_grad) && n > 0) || !dev_loss ||
+        (((uintptr_t)dev_img | (uintptr_t)dev_target | (uintptr_t)dev_grad) & 15u)) {
+        set_error(ctx, "fsx_l2_loss: null or misaligned buffer");
+        return 1;
+    }
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    if (!accumulate)
+        HIPCHECK(ctx, fsk_zero_f32(dev_loss, 1, ctx->stream));
+    HIPCHECK(ctx, fsk_l2_loss(dev_img, dev_target, (size_t)n, dev_grad, dev_loss, ctx->stream));
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Batched renders                                                     */
 /* ------------------------------------------------------------------ */

@@ -37,9 +37,6 @@ typedef struct {
     uint32_t sky;
 } fsk_frame;
 
-/* Slice table: n records of {start_x, start_y, dx, dy} then n inv_z. */
-int fsk_slices(const fsk_frame *f, float *d_tab, hipStream_t s);
-
 /* Plain render of one frame into out[rows][pitch]; the kernel builds its
  * own slice table per depth chunk. */
 int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
@@ -112,6 +109,8 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
                   int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
                   float *d_gsky, hipStream_t s);
 int fsk_zero_f32(float *d_p, size_t n, hipStream_t s);
+int fsk_l2_loss(const float *d_img, const float *d_tgt, size_t n, float *d_g, float *d_loss,
+                hipStream_t s);
 int fsk_shade_factor(const uint16_t *d_hplain, int32_t hrows, int32_t hcols, float sun0,
                      float sun1, float sun2, int32_t size, float *d_sf, hipStream_t s);
 

@@ -226,16 +226,6 @@ __device__ __forceinline__ Slice slice_params(const fsk_frame &f, int32_t k)
     return s;
 }
 
-__global__ void k_slices(fsk_frame f, float4 *__restrict__ lines, float *__restrict__ invz)
-{
-    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= f.n)
-        return;
-    Slice s = slice_params(f, k);
-    lines[k] = make_float4(s.sx, s.sy, s.dx, s.dy);
-    invz[k] = s.iz;
-}
-
 /* ------------------------------------------------------------------ */
 /* The render kernel                                                   */
 /* ------------------------------------------------------------------ */
@@ -1254,24 +1244,78 @@ __device__ __forceinline__ bool segment_sum3(uint32_t key, float v[3])
     return lane == 0 || prev != key;
 }
 
-constexpr int kGradRows = 8;           /* rows per thread in the adjoint */
+constexpr int kGradThreads = 1024;
+constexpr int kGradRows = 2;           /* rows per thread in the adjoint */
+constexpr int kGradBlockRows = (kGradThreads / 64) * kGradRows;   /* 32 */
+constexpr int kGradCapLog = 12;
+constexpr int kGradCap = 1 << kGradCapLog;   /* texel slots of a workgroup's LDS table */
+constexpr int kGradProbes = 8;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
-/* Adjoint.  Each thread holds kGradRows rows of one column; a wave is 64
- * adjacent columns of the same rows.  A run of equal src down the column
- * is one visible sample: its incoming gradient is summed in registers and
- * pushed through the three mixes once, when the run ends.  For
- * px = mix(wy0, X1, wy1, X2), X1 = mix(wx0, s0, wx1, s1):
- * d px / d s0 = wy0' * wx0' * s0 / px (X1 cancels), likewise for the other
- * texels, 0 where px == 0.  Before the f32 atomics, lanes of the wave that
- * end a run on the same row and hit the same texel (adjacent columns
- * magnify the same near texel) are summed by a segmented reduction, so a
- * hot texel receives one atomic per wave instead of one per column.  The
- * sky term is reduced per wave. */
-__global__ __launch_bounds__(256) void k_diff_grad(DiffArgs a)
+/* Add v[0..2] to texel `key` of the workgroup's LDS table (open addressing,
+ * linear probing).  A texel that finds no slot within kGradProbes, or any
+ * texel once the table is three quarters full (far-range blocks, where
+ * every sample has texels of its own), goes straight to the global
+ * gradient. */
+__device__ __forceinline__ void table_add(uint32_t *keys, float *vals, uint32_t *fill, float *gcp,
+                                          size_t plane, uint32_t key, const float v[3])
 {
+    uint32_t h = (key * 2654435761u) >> (32 - kGradCapLog);
+    const bool open = *(volatile uint32_t *)fill < (uint32_t)(kGradCap * 3 / 4);
+#pragma unroll 1
+    for (int p = 0; open && p < kGradProbes; p++) {
+        uint32_t k = keys[h];
+        if (k == kNoKey) {
+            k = atomicCAS(&keys[h], kNoKey, key);
+            if (k == kNoKey)
+                atomicAdd(fill, 1u);
+        }
+        if (k == kNoKey || k == key) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++)
+                if (v[ch] != 0.0f)
+                    atomicAdd(&vals[ch * kGradCap + h], v[ch]);
+            return;
+        }
+        h = (h + 1) & (kGradCap - 1);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++)
+        if (v[ch] != 0.0f)
+            unsafeAtomicAdd(&gcp[ch * plane + key], v[ch]);
+}
+
+/* Adjoint.  A workgroup covers 64 adjacent columns x 32 rows; each thread
+ * holds kGradRows rows of one column.  A run of equal src down the column
+ * is one visible sample: its incoming gradient is summed in registers and
+ * pushed through the three mixes once.  For px = mix(wy0, X1, wy1, X2),
+ * X1 = mix(wx0, s0, wx1, s1): d px / d s0 = wy0' * wx0' * s0 / px (X1
+ * cancels), likewise for the other texels, 0 where px == 0.  The texel
+ * terms are summed in two stages before any global atomic: lanes of a
+ * wave that end a run on the same row and hit the same texel (adjacent
+ * columns magnifying one near texel) are merged by a segmented reduction
+ * across the wave, so that the LDS table below sees distinct texels per
+ * instruction; then an LDS hash table sums the terms per texel over the
+ * whole 64 x 32 block (neighbouring samples share texels: a block touches
+ * about a seventh as many texels as it has sample-texel terms).  Short
+ * per-thread row runs keep many waves in flight for the gathers.  Each
+ * table entry is then added into the global f32 gradient with one atomic
+ * per channel.  The sky term is reduced per wave. */
+__global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
+{
+    __shared__ uint32_t keys[kGradCap];
+    __shared__ float vals[3 * kGradCap];
+    __shared__ uint32_t fill;
+    for (int32_t i = threadIdx.x; i < kGradCap; i += kGradThreads) {
+        keys[i] = kNoKey;
+        vals[i] = 0.0f;
+        vals[kGradCap + i] = 0.0f;
+        vals[2 * kGradCap + i] = 0.0f;
+    }
+    if (threadIdx.x == 0)
+        fill = 0u;
     const int32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int32_t r0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * kGradRows;
+    const int32_t r0 = blockIdx.y * kGradBlockRows + (threadIdx.x >> 6) * kGradRows;
     const int32_t H = a.f.rows, W = a.f.cols;
     const bool ok = col < W && r0 < H;
     const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
@@ -1287,6 +1331,7 @@ __global__ __launch_bounds__(256) void k_diff_grad(DiffArgs a)
         for (int ch = 0; ch < 3; ch++)
             g[ch][j] = in ? a.gimg[ch * fplane + p] : 0.0f;
     }
+    __syncthreads();
     float sky_acc = 0.0f;
     float run[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -1316,31 +1361,24 @@ __global__ __launch_bounds__(256) void k_diff_grad(DiffArgs a)
                 const float wx0n = b.wx0 / wxs, wx1n = b.wx1 / wxs;
                 const float wy0n = b.wy0 / wys, wy1n = b.wy1 / wys;
                 const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
-                float sf[4];
+                float q[3];
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++)
+                    q[ch] = px[ch] > 0.0f ? run[ch] / px[ch] : 0.0f;
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    sf[t] = a.sf[b.t[t]];
+                    const float sf = a.sf[b.t[t]];
                     key[t] = (uint32_t)b.t[t];
-                }
 #pragma unroll
-                for (int ch = 0; ch < 3; ch++) {
-                    if (!(px[ch] > 0.0f))
-                        continue;
-                    const float q = run[ch] / px[ch];
-#pragma unroll
-                    for (int t = 0; t < 4; t++)
-                        val[t][ch] = ((q * wt[t]) * sv[ch][t]) * sf[t];
+                    for (int ch = 0; ch < 3; ch++)
+                        val[t][ch] = ((q[ch] * wt[t]) * sv[ch][t]) * sf;
                 }
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const bool head = segment_sum3(key[t], val[t]);
-                if (head && key[t] != kNoKey) {
-#pragma unroll
-                    for (int ch = 0; ch < 3; ch++)
-                        if (val[t][ch] != 0.0f)
-                            unsafeAtomicAdd(&a.gcp[ch * plane + key[t]], val[t][ch]);
-                }
+                if (head && key[t] != kNoKey)
+                    table_add(keys, vals, &fill, a.gcp, plane, key[t], val[t]);
             }
         }
         if (ends) {
@@ -1352,6 +1390,49 @@ __global__ __launch_bounds__(256) void k_diff_grad(DiffArgs a)
         sky_acc += __shfl_xor(sky_acc, off);
     if ((threadIdx.x & 63) == 0 && sky_acc != 0.0f)
         unsafeAtomicAdd(a.gsky, sky_acc);
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < kGradCap; i += kGradThreads) {
+        const uint32_t key = keys[i];
+        if (key == kNoKey)
+            continue;
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float v = vals[ch * kGradCap + i];
+            if (v != 0.0f)
+                unsafeAtomicAdd(&a.gcp[ch * plane + key], v);
+        }
+    }
+}
+
+/* L2 loss of the fitting harness: g = 2 * (img - target) and
+ * loss += sum((img - target)^2), one pass, one atomic per workgroup. */
+__global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
+                                                 const float *__restrict__ tgt, size_t n,
+                                                 float *__restrict__ g, float *__restrict__ loss)
+{
+    __shared__ float part[4];
+    float acc = 0.0f;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t n4 = n / 4;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        float4 a = ((const float4 *)img)[i], b = ((const float4 *)tgt)[i];
+        float4 d = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+        ((float4 *)g)[i] = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
+        acc += ((d.x * d.x) + (d.y * d.y)) + ((d.z * d.z) + (d.w * d.w));
+    }
+    for (size_t i = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float d = img[i] - tgt[i];
+        g[i] = 2.0f * d;
+        acc += d * d;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        unsafeAtomicAdd(loss, (part[0] + part[1]) + (part[2] + part[3]));
 }
 
 /* Zero fill (a kernel, not a memset node, so that callers may capture it). */
@@ -1482,6 +1563,21 @@ int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int
     return hipGetLastError();
 }
 
+int fsk_l2_loss(const float *d_img, const float *d_tgt, size_t n, float *d_g, float *d_loss,
+                hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    size_t blocks = (n / 4 + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(k_l2_loss, dim3((unsigned)blocks), dim3(256), 0, s, d_img, d_tgt, n, d_g,
+                       d_loss);
+    return hipGetLastError();
+}
+
 int fsk_zero_f32(float *d_p, size_t n, hipStream_t s)
 {
     if (n == 0)
@@ -1503,8 +1599,8 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
     a.gimg = d_gimg;
     a.gcp = d_gcp;
     a.gsky = d_gsky;
-    dim3 grid((f->cols + 63) / 64, (f->rows + 4 * kGradRows - 1) / (4 * kGradRows));
-    hipLaunchKernelGGL(k_diff_grad, grid, dim3(256), 0, s, a);
+    dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
+    hipLaunchKernelGGL(k_diff_grad, grid, dim3(kGradThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1527,16 +1623,6 @@ int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s)
     if (which < 0 || which > 3)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_selftest, dim3(4096), dim3(256), 0, s, which, d_bad);
-    return hipGetLastError();
-}
-
-int fsk_slices(const fsk_frame *f, float *d_tab, hipStream_t s)
-{
-    if (f->n <= 0)
-        return hipSuccess;
-    float4 *lines = (float4 *)d_tab;
-    float *invz = d_tab + 4 * (size_t)f->n;
-    hipLaunchKernelGGL(k_slices, dim3((f->n + 255) / 256), dim3(256), 0, s, *f, lines, invz);
     return hipGetLastError();
 }
 

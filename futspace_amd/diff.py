@@ -97,6 +97,22 @@ class DiffRenderer:
                 "fsx_render_f32_vjp")
         return gplanes, gsky
 
+    def l2_loss_grad(self, img, target, grad=None, loss=None, accumulate=False):
+        """One fused pass: grad = 2 * (img - target) and loss (a 1-element
+        tensor) = sum((img - target)^2), added into loss with accumulate."""
+        if img.shape != target.shape or not img.is_contiguous() or not target.is_contiguous():
+            raise ValueError("img and target must be contiguous tensors of one shape")
+        if grad is None:
+            grad = torch.empty_like(img)
+        if loss is None:
+            loss = torch.empty(1, dtype=torch.float32, device="cuda")
+            accumulate = False
+        with self.stream():
+            self.ctx.check(self.ctx.lib.fsx_l2_loss(
+                self.ctx.ptr, _ptr(img), _ptr(target), img.numel(), 1 if accumulate else 0,
+                _ptr(grad), _ptr(loss)), "fsx_l2_loss")
+        return grad, loss
+
     def render(self, planes, shading, view: View):
         """Differentiable render: img[3][h][w] as a function of planes and
         shading = (sun_height, sun_ang)."""
@@ -142,18 +158,17 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
     opt = torch.optim.Adam([planes], lr=lr)
     grad = torch.zeros_like(planes)
     gsky = torch.zeros(1, dtype=torch.float32, device="cuda")
-    img = None
-    src = None
+    img = gimg = src = None
     losses = torch.zeros(steps, dtype=torch.float32, device="cuda")
     for it in range(steps):
         for i, (v, t) in enumerate(zip(views, targets)):
             if img is None or tuple(img.shape) != (3, v.h, v.w):
                 img = torch.empty((3, v.h, v.w), dtype=torch.float32, device="cuda")
+                gimg = torch.empty_like(img)
                 src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
             renderer.forward_raw(planes, sky_scale, v, img, src)
-            diff = img - t
-            losses[it] += (diff * diff).sum()
-            renderer.vjp_raw(planes, src, 2.0 * diff, v, grad, gsky, accumulate=i > 0)
+            renderer.l2_loss_grad(img, t, gimg, losses[it:it + 1], accumulate=True)
+            renderer.vjp_raw(planes, src, gimg, v, grad, gsky, accumulate=i > 0)
         planes.grad = grad
         opt.step()
         if on_step is not None:

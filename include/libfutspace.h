@@ -211,6 +211,13 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
                        const int32_t *dev_src, const float *dev_gimg, int32_t accumulate,
                        float *dev_gplanes, float *dev_gsky);
 
+/* Squared-L2 loss of the fitting harness in one pass over n f32 values
+ * (16-byte aligned device buffers): dev_grad = 2 * (img - target) and
+ * *dev_loss = sum((img - target)^2), added to the old value when
+ * accumulate is nonzero.  Asynchronous on the context stream. */
+int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *dev_target,
+                int64_t n, int32_t accumulate, float *dev_grad, float *dev_loss);
+
 /* Render kernel shape, as chosen by the autotuner: columns per workgroup
  * tile (4 or 8), workgroups per CU for 4-column tiles (1 or 2) and depth
  * slices per chunk (256-2048); 0 leaves a knob automatic.  Applies to

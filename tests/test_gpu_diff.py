@@ -192,3 +192,18 @@ def test_fit_reduces_loss(ctx):
     losses = fit(r, planes, views, targets, SUN_H, steps=20, lr=0.01).cpu().numpy()
     assert np.all(np.isfinite(losses))
     assert losses[-1] < 0.5 * losses[0], losses
+
+
+def test_fused_l2_loss(ctx):
+    r, _, _, _ = _setup(ctx)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    for n in (4, 7, 3 * 97 * 131, 3 * 720 * 1280):
+        a = torch.rand(n, device="cuda", generator=gen)
+        b = torch.rand(n, device="cuda", generator=gen)
+        g, loss = r.l2_loss_grad(a, b)
+        g2, loss2 = r.l2_loss_grad(a, b, loss=loss.clone(), accumulate=True)
+        torch.cuda.synchronize()
+        want = ((a.double() - b.double()) ** 2).sum()
+        assert torch.equal(g, 2.0 * (a - b))
+        assert abs(float(loss[0]) - float(want)) <= 1e-5 * float(want)
+        assert abs(float(loss2[0]) - 2 * float(want)) <= 2e-5 * float(want)
