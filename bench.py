@@ -42,6 +42,23 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_render_plain_2048_1080p.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_render launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the
+    counter reads for a 64 MiB copy); None if the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def calib_copy():
     """A plain device-to-device copy of known size: 64 MiB read + 64 MiB
     written per launch, the yardstick for FETCH_SIZE / WRITE_SIZE."""
@@ -325,7 +342,9 @@ def main():
         "gpix_per_s": fps * H * W / 1e9,
         "render_kernel_ms": kern_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(),
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
                      "kernel": "k_render", "algorithmic_bytes_per_launch": bytes_per_frame},
     }
     if not args.no_batch:

@@ -230,10 +230,14 @@ __device__ __forceinline__ Slice slice_params(const fsk_frame &f, int32_t k)
 /* The render kernel                                                   */
 /* ------------------------------------------------------------------ */
 
-constexpr int kThreads = 1024;
-constexpr int kWaves = kThreads / 64;            /* 16 */
+constexpr int kMaxWaves = 16;                    /* 1024-thread workgroups */
 constexpr int kChunk = 2048;                     /* slices per chunk, at most */
-constexpr int kTabStride = kChunk + kChunk / 16; /* skewed slice-table entries */
+
+/* Skewed slice-table entries for chunks of up to `chunk` slices. */
+__host__ __device__ constexpr int tab_stride(int chunk)
+{
+    return chunk + chunk / 16;
+}
 constexpr int kMaxRows = 4096;                   /* queue entries keep the row in 12 bits */
 
 /* Slice s of a chunk lives at s + s/16: the 16 slice groups of a wave
@@ -391,13 +395,17 @@ __device__ __forceinline__ uint32_t color_bilinear_lut(const fsk_map &m, const f
 
 constexpr int kShortSpan = 24;   /* span rows a lane paints itself */
 
+/* Dynamic LDS of a tile render: slice table (float4 + inv_z per skewed
+ * entry), the TC x rows colour stripe, the visible-sample queue and span
+ * bottoms (at most min(rows, chunk) samples per column and chunk) and the
+ * long-span list. */
 template <int TC>
-size_t render_lds_bytes_tc(int32_t rows)
+size_t render_lds_bytes_tc(int32_t rows, int32_t chunk)
 {
     size_t hp = (size_t)((rows + 3) & ~3);
-    size_t q = (size_t)TC * (size_t)(rows < kChunk ? rows : kChunk);
+    size_t q = (size_t)TC * (size_t)(rows < chunk ? rows : chunk);
     size_t longs = (size_t)TC * (size_t)(rows / kShortSpan + 1);
-    return sizeof(float) * 5 * kTabStride +
+    return sizeof(float) * 5 * (size_t)tab_stride(chunk) +
            sizeof(uint32_t) * (hp * TC + q + (q + 1) / 2 + 2 * longs);
 }
 
@@ -420,8 +428,8 @@ size_t render_lds_bytes_tc(int32_t rows)
 template <int TC>
 struct TileShared {
     float sq[256];                 /* squares table, filled once per workgroup */
-    int32_t wtot[kWaves][TC];
-    int32_t wpre[kWaves][TC];
+    int32_t wtot[kMaxWaves][TC];
+    int32_t wpre[kMaxWaves][TC];
     int32_t carry[TC];
     uint32_t qn, nlong, transparent;
 };
@@ -442,19 +450,24 @@ __device__ __forceinline__ uint32_t tile_value(uint32_t v, uint32_t sky)
     return v ? v - 1u : 0xFFFFFFFFu;
 }
 
-template <int TC, int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
+/* chunk_max must be a multiple of NT/TC (the slice groups) and at most
+ * kChunk; the LDS layout is sized for it. */
+template <int TC, int NT, int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
 __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f, uint32_t *out,
                                             int64_t pitch, int32_t tile, int32_t chunk_max,
                                             uint32_t *lds, TileShared<TC> &sh)
 {
-    constexpr int SG = kThreads / TC;            /* slice groups */
+    constexpr int NW = NT / 64;                  /* waves */
+    constexpr int SG = NT / TC;                  /* slice groups */
     constexpr int KR = kChunk / SG;              /* slices per lane per chunk, at most */
-    float4 *tl = (float4 *)lds;                                  /* [kTabStride] */
-    float *tz = (float *)(lds + 4 * kTabStride);                 /* [kTabStride] */
+    static_assert(NW <= kMaxWaves && KR % 2 == 0 && KR <= 32, "tile shape");
+    const int32_t tabs = tab_stride(chunk_max);
+    float4 *tl = (float4 *)lds;                                  /* [tabs] */
+    float *tz = (float *)(lds + 4 * tabs);                       /* [tabs] */
     const int32_t H = f.rows, n = f.n;
     const int32_t Hp = (H + 3) & ~3;
-    uint32_t *stripe = lds + 5 * kTabStride;                     /* [TC][Hp] */
-    const int32_t qcap = TC * min(H, kChunk);                    /* visible rows differ */
+    uint32_t *stripe = lds + 5 * tabs;                           /* [TC][Hp] */
+    const int32_t qcap = TC * min(H, chunk_max);                 /* visible rows differ */
     uint32_t *queue = stripe + (size_t)Hp * TC;                  /* [qcap] */
     uint16_t *qbot = (uint16_t *)(queue + qcap);                 /* [qcap] span bottoms */
     uint32_t *longs = queue + qcap + (qcap + 1) / 2;             /* [2][TC*(H/24+1)] */
@@ -467,7 +480,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     const float fcol = (float)col;
     const float Hf = (float)H;
 
-    for (int32_t i = tid; i < Hp * TC; i += kThreads)
+    for (int32_t i = tid; i < Hp * TC; i += NT)
         stripe[i] = 0u;
     if (tid < TC)
         sh.carry[tid] = H;
@@ -483,7 +496,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     for (int32_t base = 0; base < n; base += csz) {
         const int32_t cn = min(csz, n - base);
         /* phase 0 */
-        for (int32_t s = tid; s < cn; s += kThreads) {
+        for (int32_t s = tid; s < cn; s += NT) {
             Slice sl = slice_params(f, base + s);
             int32_t o = skew(s);
             tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
@@ -532,11 +545,11 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         __syncthreads();
         if (wave == 0) {
             /* lane = w*4 + j handles column j (and j+4 when TC == 8) of
-             * wave w: exclusive prefix over the 16 waves */
+             * wave w: exclusive prefix over the NW waves */
 #pragma unroll
             for (int half = 0; half < TC / 4; half++) {
                 int32_t cc = (lane & 3) + 4 * half;
-                int32_t t = sh.wtot[lane >> 2][cc];
+                int32_t t = (lane >> 2) < NW ? sh.wtot[lane >> 2][cc] : H;
 #pragma unroll
                 for (int off = 4; off < 64; off <<= 1) {
                     int32_t o = __shfl_up(t, off);
@@ -544,7 +557,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                         t = min(t, o);
                 }
                 int32_t ex = __shfl_up(t, 4);
-                sh.wpre[lane >> 2][cc] = min(sh.carry[cc], lane < 4 ? H : ex);
+                if ((lane >> 2) < NW)
+                    sh.wpre[lane >> 2][cc] = min(sh.carry[cc], lane < 4 ? H : ex);
             }
         }
         __syncthreads();
@@ -603,7 +617,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 
         /* phase 4: colour and span */
         const uint32_t nq = sh.qn;
-        for (uint32_t i = tid; i < nq; i += kThreads) {
+        for (uint32_t i = tid; i < nq; i += NT) {
             uint32_t e = queue[i];
             int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
             int32_t row = (int32_t)(e & 0xFFFu);
@@ -643,7 +657,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         uint32_t d = longs[2 * li], colour = longs[2 * li + 1];
         int32_t cc = (int32_t)(d >> 26), r0 = (int32_t)((d >> 13) & 0x1FFFu);
         int32_t r1 = (int32_t)(d & 0x1FFFu);
-        for (int32_t r = r0 + tid; r < r1; r += kThreads)
+        for (int32_t r = r0 + tid; r < r1; r += NT)
             stripe[cc * Hp + r] = colour;
     }
     if (nl)
@@ -657,7 +671,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
          * TC*4-byte row segments */
         constexpr int PARTS = TC / 4;
         const int32_t part = tid % PARTS;
-        for (int32_t r = tid / PARTS; r < H; r += kThreads / PARTS) {
+        for (int32_t r = tid / PARTS; r < H; r += NT / PARTS) {
             uint32_t px[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -680,7 +694,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     /* phase 5 with sh.transparent samples: a zero cell takes the last
      * non-zero colour above it (fill_vline), else the sky; each thread
      * owns a block of consecutive rows */
-    const int32_t per = (H + kThreads - 1) / kThreads;
+    const int32_t per = (H + NT - 1) / NT;
     const int32_t r0 = min(tid * per, H), r1 = min(r0 + per, H);
     uint32_t pre[TC];
 #pragma unroll
@@ -692,7 +706,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         }
         pre[j] = p;
     }
-    uint32_t *wfill = (uint32_t *)&sh.wtot[0][0];      /* [kWaves][TC] */
+    uint32_t *wfill = (uint32_t *)&sh.wtot[0][0];      /* [NW][TC] */
     uint32_t *wfpre = (uint32_t *)&sh.wpre[0][0];
 #pragma unroll
     for (int j = 0; j < TC; j++) {
@@ -713,7 +727,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 #pragma unroll
         for (int half = 0; half < TC / 4; half++) {
             int32_t cc = (lane & 3) + 4 * half;
-            uint32_t t = wfill[(lane >> 2) * TC + cc];
+            uint32_t t = (lane >> 2) < NW ? wfill[(lane >> 2) * TC + cc] : 0u;
 #pragma unroll
             for (int off = 4; off < 64; off <<= 1) {
                 uint32_t o = __shfl_up(t, off);
@@ -721,7 +735,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                     t = t ? t : o;
             }
             uint32_t ex = __shfl_up(t, 4);
-            wfpre[(lane >> 2) * TC + cc] = lane < 4 ? 0u : ex;
+            if ((lane >> 2) < NW)
+                wfpre[(lane >> 2) * TC + cc] = lane < 4 ? 0u : ex;
         }
     }
     __syncthreads();
@@ -752,14 +767,15 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     }
 }
 
-template <int TC, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
-__global__ __launch_bounds__(kThreads, WPE) void k_render(RenderArgs a)
+template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
+__global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
 {
     extern __shared__ uint32_t lds[];
     __shared__ TileShared<TC> sh;
     fill_square_lut(sh.sq, threadIdx.x);
-    render_tile<TC, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
-                                           xcd_tile(blockIdx.x, a.ntiles), a.chunk_max, lds, sh);
+    render_tile<TC, NT, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
+                                               xcd_tile(blockIdx.x, a.ntiles), a.chunk_max, lds,
+                                               sh);
 }
 
 /* ------------------------------------------------------------------ */
@@ -879,10 +895,10 @@ __global__ void k_selftest(int which, unsigned long long *bad)
         atomicAdd(bad, local);
 }
 
-template <int TC, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
+template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
 hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
 {
-    auto kern = k_render<TC, HBITS, HP2, CP2, WPE, MODE>;
+    auto kern = k_render<TC, NT, HBITS, HP2, CP2, WPE, MODE>;
     static int dyn_max = -1;
     if (dyn_max < 0) {
         /* the whole LDS of a CU minus the kernel's static shared memory */
@@ -901,10 +917,10 @@ hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
             return e;
         dyn_max = want;
     }
-    size_t need = render_lds_bytes_tc<TC>(a.f.rows);
+    size_t need = render_lds_bytes_tc<TC>(a.f.rows, a.chunk_max);
     if (need > (size_t)dyn_max)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(kern, dim3(a.ntiles), dim3(kThreads), need, s, a);
+    hipLaunchKernelGGL(kern, dim3(a.ntiles), dim3(NT), need, s, a);
     return hipGetLastError();
 }
 
@@ -913,10 +929,11 @@ hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
 /* ------------------------------------------------------------------ */
 
 /* Render kernel shape: columns per workgroup tile (8: 32-byte row
- * segments, one 1024-thread workgroup per CU; 4: 16-byte segments, for
- * frames whose 8-column stripe does not fit in LDS), for 4 columns the
- * occupancy the registers are budgeted for (1 or 2 workgroups per CU), and
- * the depth-chunk length.  fsk_set_tuning (the autotuner) or the
+ * segments, one 1024-thread workgroup per CU; 4: 16-byte segments), for 4
+ * columns one 1024-thread or two 512-thread workgroups per CU (the second
+ * lets one tile's barrier and gather stalls overlap the other's work; it
+ * needs the LDS of two tiles, so its depth chunk defaults to 1024), and the
+ * depth-chunk length.  fsk_set_tuning (the autotuner) or the
  * FSX_RENDER_TC / FSX_RENDER_WG_PER_CU / FSX_RENDER_CHUNK environment
  * variables choose them; 0 means automatic. */
 fsk_tuning g_tuning = {-1, -1, -1};
@@ -963,17 +980,46 @@ inline int cu_count()
 }
 
 struct Shape {
-    int tc, wpe, chunk;
+    int tc, nt, wg, chunk;
 };
+
+constexpr size_t kStaticLds = 4096;   /* headroom for the kernels' static shared memory */
 
 inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
 {
-    /* 4 KB of headroom for the kernels' static shared memory */
-    bool fits8 = render_lds_bytes_tc<8>(rows) + 4096 <= lds_per_cu();
     Shape s;
-    s.tc = (t.tc == 4 || !fits8) ? 4 : 8;
-    s.wpe = (s.tc == 4 && t.wg_per_cu == 2) ? 8 : 4;
-    s.chunk = (t.chunk >= 256 && t.chunk <= kChunk) ? t.chunk : kChunk;
+    int chunk = (t.chunk >= 256 && t.chunk <= kChunk) ? t.chunk : 0;
+    if (t.tc != 4 && render_lds_bytes_tc<8>(rows, chunk ? chunk : kChunk) + kStaticLds <=
+                         lds_per_cu()) {
+        s.tc = 8;
+        s.nt = 1024;
+        s.wg = 1;
+        s.chunk = chunk ? chunk : kChunk;
+    } else {
+        s.tc = 4;
+        int c2 = chunk ? chunk : 1024;
+        c2 -= c2 % 128;
+        if (t.wg_per_cu == 2 && 2 * (render_lds_bytes_tc<4>(rows, c2) + kStaticLds) <=
+                                    lds_per_cu()) {
+            s.nt = 512;
+            s.wg = 2;
+            s.chunk = c2;
+        } else {
+            s.nt = 1024;
+            s.wg = 1;
+            s.chunk = chunk ? chunk : kChunk;
+        }
+    }
+    /* a chunk holds whole slice groups; tall frames take shorter chunks
+     * until the tile fits in LDS */
+    int sg = s.nt / s.tc;
+    s.chunk = s.chunk < sg ? sg : s.chunk - s.chunk % sg;
+    auto need = [&](int c) {
+        return (s.tc == 8 ? render_lds_bytes_tc<8>(rows, c) : render_lds_bytes_tc<4>(rows, c)) +
+               kStaticLds;
+    };
+    while (s.chunk > sg && (size_t)s.wg * need(s.chunk) > lds_per_cu())
+        s.chunk = (s.chunk / 2) - (s.chunk / 2) % sg;
     return s;
 }
 
@@ -984,11 +1030,11 @@ hipError_t launch_render(RenderArgs a, hipStream_t s)
     a.chunk_max = sh.chunk;
     if (sh.tc == 8) {
         a.ntiles = (a.f.cols + 7) / 8;
-        return launch_render_cfg<8, HBITS, HP2, CP2, 4, MODE>(a, s);
+        return launch_render_cfg<8, 1024, HBITS, HP2, CP2, 4, MODE>(a, s);
     }
     a.ntiles = (a.f.cols + 3) / 4;
-    return sh.wpe == 8 ? launch_render_cfg<4, HBITS, HP2, CP2, 8, MODE>(a, s)
-                       : launch_render_cfg<4, HBITS, HP2, CP2, 4, MODE>(a, s);
+    return sh.nt == 512 ? launch_render_cfg<4, 512, HBITS, HP2, CP2, 4, MODE>(a, s)
+                        : launch_render_cfg<4, 1024, HBITS, HP2, CP2, 4, MODE>(a, s);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1022,8 +1068,8 @@ __global__ void k_batch_reset(uint32_t *counters, int32_t n)
  * neighbouring tiles of a view stay on one L2; a workgroup whose own queue
  * is drained takes from the next queues.  Every workgroup exits once all
  * queues report exhausted, after at most nqueues claims. */
-template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
-__global__ __launch_bounds__(kThreads, WPE) void k_render_batch(BatchArgs b)
+template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE>
+__global__ __launch_bounds__(NT, WPE) void k_render_batch(BatchArgs b)
 {
     extern __shared__ uint32_t lds[];
     __shared__ TileShared<TC> sh;
@@ -1051,15 +1097,15 @@ __global__ __launch_bounds__(kThreads, WPE) void k_render_batch(BatchArgs b)
         if (it == 0xFFFFFFFFu)
             break;
         const fsk_batch_view &bv = b.views[it >> 16];
-        render_tile<TC, HBITS, HP2, CP2>(bv.m, bv.f, b.out + bv.offset, bv.f.cols, it & 0xFFFFu,
-                                         b.chunk_max, lds, sh);
+        render_tile<TC, NT, HBITS, HP2, CP2>(bv.m, bv.f, b.out + bv.offset, bv.f.cols,
+                                             it & 0xFFFFu, b.chunk_max, lds, sh);
     }
 }
 
-template <int TC, int HBITS, bool HP2, bool CP2, int WPE>
+template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE>
 hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
 {
-    auto kern = k_render_batch<TC, HBITS, HP2, CP2, WPE>;
+    auto kern = k_render_batch<TC, NT, HBITS, HP2, CP2, WPE>;
     static int dyn_max = -1;
     if (dyn_max < 0) {
         hipFuncAttributes fa;
@@ -1072,7 +1118,7 @@ hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
             return e;
         dyn_max = want;
     }
-    size_t need = render_lds_bytes_tc<TC>(fb->max_rows);
+    size_t need = render_lds_bytes_tc<TC>(fb->max_rows, chunk);
     if (need > (size_t)dyn_max)
         return hipErrorInvalidValue;
     BatchArgs b;
@@ -1084,7 +1130,7 @@ hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
     b.nqueues = fb->nqueues;
     b.chunk_max = chunk;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, fb->counters, fb->nqueues);
-    hipLaunchKernelGGL(kern, dim3(fb->grid), dim3(kThreads), need, s, b);
+    hipLaunchKernelGGL(kern, dim3(fb->grid), dim3(NT), need, s, b);
     return hipGetLastError();
 }
 
@@ -1093,9 +1139,9 @@ hipError_t launch_batch(const fsk_batch *fb, hipStream_t s)
 {
     Shape sh = resolve_shape(fb->max_rows, fb->tune);
     if (sh.tc == 8)
-        return launch_batch_cfg<8, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
-    return sh.wpe == 8 ? launch_batch_cfg<4, HBITS, HP2, CP2, 8>(fb, sh.chunk, s)
-                       : launch_batch_cfg<4, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
+        return launch_batch_cfg<8, 1024, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
+    return sh.nt == 512 ? launch_batch_cfg<4, 512, HBITS, HP2, CP2, 4>(fb, sh.chunk, s)
+                        : launch_batch_cfg<4, 1024, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1479,8 +1525,9 @@ extern "C" {
 
 size_t fsk_render_lds_bytes(int32_t rows)
 {
-    return resolve_shape(rows, current_tuning()).tc == 8 ? render_lds_bytes_tc<8>(rows)
-                                                         : render_lds_bytes_tc<4>(rows);
+    Shape sh = resolve_shape(rows, current_tuning());
+    return sh.tc == 8 ? render_lds_bytes_tc<8>(rows, sh.chunk)
+                      : render_lds_bytes_tc<4>(rows, sh.chunk);
 }
 
 int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t)
@@ -1491,7 +1538,7 @@ int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t)
 int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t)
 {
     Shape sh = resolve_shape(max_rows, *t);
-    return cu_count() * (sh.wpe == 8 ? 2 : 1);
+    return cu_count() * sh.wg;
 }
 
 void fsk_set_tuning(const fsk_tuning *t)

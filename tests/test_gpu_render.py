@@ -163,3 +163,23 @@ def test_device_arithmetic_selftest(ctx, which):
     ctx.check(ctx.lib.fsx_selftest(ctx.ptr, which, ctypes.byref(bad), ctypes.byref(first)),
               "fsx_selftest")
     assert bad.value == 0, f"{bad.value} mismatches, first input bits {first.value:#x}"
+
+
+def test_reference_map_data_bitexact(ctx):
+    """The reference's C1W/D1 maps (a 256^2 crop packed as the reference
+    host loads them, tests/golden/maps.npz)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "maps.npz"))
+    col, hts = g["c1w_colour"], g["d1_height"]
+    m = ctx.map(col, hts, shade_size=256)
+    shaded, alt = O.update_map(col, hts, size=256)
+    _check(m.shaded(), shaded)
+    for v in [View(x=128.5, y=200.25, height=130.0, angle=0.3, horizon=10.0, distance=300.0,
+                   fov=1.2, sky_color=sky_color(0.1), h=120, w=160, d=0.004, z0=0.3),
+              View(x=40.7, y=31.1, height=140.0, angle=2.9, horizon=-20.0, distance=500.0,
+                   fov=1.2, sky_color=sky_color(0.1), h=90, w=200, d=0.002, z0=0.5, scale=240.0),
+              View(x=250.0, y=128.0, height=110.0, angle=-1.2, horizon=30.0, distance=250.0,
+                   fov=0.9, sky_color=sky_color(0.1), h=150, w=100, d=0.01, z0=1.0)]:
+        want = O.render_off(v, shaded, alt)
+        assert (want == v.sky_color).any() and (want != v.sky_color).any()
+        _check(ctx.render(m, v), want)

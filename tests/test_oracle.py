@@ -343,3 +343,48 @@ def test_relaxation_vjp_matches_finite_differences():
     assert gsky == pytest.approx(float((g[:, src < 0] * sky[:, None]).sum()), rel=1e-5)
     fd = (loss(planes, 0.2 + h) - loss(planes, 0.2 - h)) / (2 * h)
     assert gsky == pytest.approx(fd, rel=1e-3)
+
+
+# ---- the reference's own map data (tests/golden/maps.npz) ---------------
+
+def _golden():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "maps.npz"))
+    return g["c1w_colour"], g["d1_height"], g["flat_height"]
+
+
+def test_golden_maps_are_packed_like_the_reference_host():
+    colour, height, flat = _golden()
+    assert colour.shape == height.shape == (256, 256)
+    assert np.all(colour >> 24 == 0xFF)
+    h = height.view(np.uint32)
+    assert np.all(h >> 24 == 0xFF)              # FreeImage packing of the height PNG
+    assert not np.any(flat.view(np.uint32) & 0xFF)   # flat_height.png is flat at 0
+
+
+# D1's heights in this crop span 0..91: low cameras and horizons
+GOLDEN_VIEWS = [
+    View(x=128.5, y=200.25, height=130.0, angle=0.3, horizon=10.0, distance=300.0, fov=1.2,
+         sky_color=0xFF9090E0, h=120, w=160, d=0.004, z0=0.3),
+    View(x=40.7, y=31.1, height=140.0, angle=2.9, horizon=-20.0, distance=500.0, fov=1.2,
+         sky_color=0xFF9090E0, h=90, w=200, d=0.002, z0=0.5, scale=240.0),
+    View(x=250.0, y=128.0, height=110.0, angle=-1.2, horizon=30.0, distance=250.0, fov=0.9,
+         sky_color=0xFF9090E0, h=150, w=100, d=0.01, z0=1.0),
+]
+
+
+@pytest.mark.parametrize("vi", range(3))
+def test_golden_map_renders_agree(vi):
+    colour, height, _ = _golden()
+    shaded, alt = O.update_map(colour, height, size=256)
+    assert alt.max() <= 255 and alt.min() >= 0
+    v = GOLDEN_VIEWS[vi]
+    a = O.render_off(v, shaded, alt)
+    assert np.array_equal(a, O.render_painter(v, shaded, alt))
+    assert (a != v.sky_color).any() and (a == v.sky_color).any()
+
+
+def test_golden_flat_map_has_no_shadow():
+    colour, _, flat = _golden()
+    shaded, alt = O.update_map(colour, flat, size=64)
+    assert not O.shadow_counts(alt, O.sun(0.1, 0.1), 64).any()
