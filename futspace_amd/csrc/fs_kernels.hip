@@ -161,22 +161,29 @@ __device__ __forceinline__ uint32_t argb_mix(float m1, uint32_t c1, float m2, ui
 /* png_height_filtered, fut/render_functions.fut:67-77, from the height
  * quad at (floor y, floor x). */
 template <int HBITS, bool HP2>
-__device__ __forceinline__ float height_bilinear(const fsk_map &m, float x, float y)
+__device__ __forceinline__ uint2 height_quad(const fsk_map &m, float x, float y)
+{
+    int32_t ix = wrap<HP2>(idx_of(floorf(x)), m.hcols);
+    int32_t iy = wrap<HP2>(idx_of(floorf(y)), m.hrows);
+    /* maps are at most 2^28 texels (host-checked), so 32-bit offsets; the
+     * wrapped index is always inside the map, whatever x and y are */
+    uint32_t t = (uint32_t)iy * (uint32_t)m.hcols + (uint32_t)ix;
+    if (HBITS == 8)
+        return make_uint2(*(const uint32_t *)((const char *)m.hq + t * 4u), 0u);
+    return *(const uint2 *)((const char *)m.hq + t * 8u);
+}
+
+template <int HBITS>
+__device__ __forceinline__ float height_interp(uint2 q, float x, float y)
 {
     float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
-    int32_t ix = wrap<HP2>(idx_of(fx), m.hcols);
-    int32_t iy = wrap<HP2>(idx_of(fy), m.hrows);
-    /* maps are at most 2^28 texels (host-checked), so 32-bit offsets */
-    uint32_t t = (uint32_t)iy * (uint32_t)m.hcols + (uint32_t)ix;
     float h00, h01, h10, h11;
     if (HBITS == 8) {
-        uint32_t q = *(const uint32_t *)((const char *)m.hq + t * 4u);
-        h00 = (float)(q & 255u);
-        h01 = (float)((q >> 8) & 255u);
-        h10 = (float)((q >> 16) & 255u);
-        h11 = (float)(q >> 24);
+        h00 = (float)(q.x & 255u);
+        h01 = (float)((q.x >> 8) & 255u);
+        h10 = (float)((q.x >> 16) & 255u);
+        h11 = (float)(q.x >> 24);
     } else {
-        uint2 q = *(const uint2 *)((const char *)m.hq + t * 8u);
         h00 = (float)(q.x & 65535u);
         h01 = (float)(q.x >> 16);
         h10 = (float)(q.y & 65535u);
@@ -346,13 +353,17 @@ __device__ __forceinline__ uint32_t color_fast(uint4 q, const float *sq, float w
  * another sum (a coordinate within 1 of the map origin); such waves, and
  * waves with translucent texels, take the general path below. */
 template <bool CP2>
-__device__ __forceinline__ uint32_t color_bilinear_lut(const fsk_map &m, const float *sq,
-                                                       float x, float y)
+__device__ __forceinline__ uint4 color_quad(const fsk_map &m, float x, float y)
+{
+    int32_t ix = wrap<CP2>(idx_of(floorf(x)), m.ccols);
+    int32_t iy = wrap<CP2>(idx_of(floorf(y)), m.crows);
+    return ((const uint4 *)m.cq)[(size_t)iy * (size_t)m.ccols + (size_t)ix];
+}
+
+/* The colour of a sample at (x, y) from its texel quad q. */
+__device__ __forceinline__ uint32_t color_of_quad(uint4 q, const float *sq, float x, float y)
 {
     float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
-    int32_t ix = wrap<CP2>(idx_of(fx), m.ccols);
-    int32_t iy = wrap<CP2>(idx_of(fy), m.crows);
-    uint4 q = ((const uint4 *)m.cq)[(size_t)iy * (size_t)m.ccols + (size_t)ix];
     float wx0 = cx - x, wx1 = x - fx, wy0 = cy - y, wy1 = y - fy;
     bool easy = (fabsf(x) >= 1.0f) && (fabsf(y) >= 1.0f) &&
                 (((q.x & q.y & q.z & q.w) >> 24) == 255u);
@@ -394,6 +405,8 @@ __device__ __forceinline__ uint32_t color_bilinear_lut(const fsk_map &m, const f
 }
 
 constexpr int kShortSpan = 24;   /* span rows a lane paints itself */
+constexpr int kGather = 8;       /* height gathers a lane keeps in flight */
+constexpr int kColourBatch = 4;  /* visible samples a lane colours per pass */
 
 /* Dynamic LDS of a tile render: slice table (float4 + inv_z per skewed
  * entry), the TC x rows colour stripe, the visible-sample queue and span
@@ -452,10 +465,13 @@ __device__ __forceinline__ uint32_t tile_value(uint32_t v, uint32_t sky)
 
 /* chunk_max must be a multiple of NT/TC (the slice groups) and at most
  * kChunk; the LDS layout is sized for it. */
-template <int TC, int NT, int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
+template <int TC, int NT, int HBITS, bool HP2, bool CP2, int MODE = kOutColour,
+          int GATHER = kGather, int CBATCH = kColourBatch>
 __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f, uint32_t *out,
                                             int64_t pitch, int32_t tile, int32_t chunk_max,
-                                            uint32_t *lds, TileShared<TC> &sh)
+                                            uint32_t *lds, TileShared<TC> &sh,
+                                            const float4 *gtl = nullptr,
+                                            const float *gtz = nullptr)
 {
     constexpr int NW = NT / 64;                  /* waves */
     constexpr int SG = NT / TC;                  /* slice groups */
@@ -495,38 +511,66 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 
     for (int32_t base = 0; base < n; base += csz) {
         const int32_t cn = min(csz, n - base);
-        /* phase 0 */
+        /* phase 0: the chunk's slice records, computed here or copied
+         * from the view's precomputed table (gtl/gtz, same values) */
         for (int32_t s = tid; s < cn; s += NT) {
-            Slice sl = slice_params(f, base + s);
             int32_t o = skew(s);
-            tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
-            tz[o] = sl.iz;
+            if (gtl) {
+                tl[o] = gtl[base + s];
+                tz[o] = gtz[base + s];
+            } else {
+                Slice sl = slice_params(f, base + s);
+                tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
+                tz[o] = sl.iz;
+            }
         }
         if (tid == 0)
             sh.qn = 0u;
         __syncthreads();
 
-        /* phase 1: rows are < 2^16, two per register */
+        /* phase 1: rows are < 2^16, two per register.  Branch-free in
+         * groups of GATHER samples: all of a group's height gathers are
+         * issued before the first is used (samples past the chunk or the
+         * frame read a valid table entry and an in-map texel, then count
+         * as row H) */
         uint32_t ah[KR / 2];
         int32_t lmin = H;
 #pragma unroll
-        for (int j = 0; j < KR; j++) {
-            int32_t s = g * kr + j;
-            int32_t r = H;
-            if (j < kr && s < cn && colok) {
-                int32_t o = skew(s);
-                float4 L = tl[o];
-                float x = L.x + (fcol * L.z);
-                float y = L.y + (fcol * L.w);
-                float hgt = height_bilinear<HBITS, HP2>(m, x, y);
-                float rel = ((f.cam_h - hgt) * tz[o]) + f.horizon;
-                r = screen_row_clamped(rel, Hf);
+        for (int j0 = 0; j0 < KR; j0 += GATHER) {
+            /* kr is uniform over the workgroup: whole groups past it are
+             * skipped, their rows stay H */
+            if (j0 >= kr) {
+#pragma unroll
+                for (int u = 0; u < GATHER; u += 2)
+                    ah[(j0 + u) / 2] = (uint32_t)H | ((uint32_t)H << 16);
+                continue;
             }
-            if (j & 1)
-                ah[j / 2] |= (uint32_t)r << 16;
-            else
-                ah[j / 2] = (uint32_t)r;
-            lmin = min(lmin, r);
+            float xs[GATHER], ys[GATHER], izs[GATHER];
+            uint2 qs[GATHER];
+#pragma unroll
+            for (int u = 0; u < GATHER; u++) {
+                int32_t o = skew(min(g * kr + j0 + u, cn - 1));
+                float4 L = tl[o];
+                xs[u] = L.x + (fcol * L.z);
+                ys[u] = L.y + (fcol * L.w);
+                izs[u] = tz[o];
+                qs[u] = height_quad<HBITS, HP2>(m, xs[u], ys[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < GATHER; u++) {
+                const int j = j0 + u;
+                float hgt = height_interp<HBITS>(qs[u], xs[u], ys[u]);
+                float rel = ((f.cam_h - hgt) * izs[u]) + f.horizon;
+                /* rows are at most H: an invalid sample is raised to H
+                 * arithmetically, so that no branch splits the group */
+                const bool ok = j < kr && g * kr + j < cn && colok;
+                int32_t r = max(screen_row_clamped(rel, Hf), ok ? 0 : H);
+                if (j & 1)
+                    ah[j / 2] |= (uint32_t)r << 16;
+                else
+                    ah[j / 2] = (uint32_t)r;
+                lmin = min(lmin, r);
+            }
         }
 
         /* phase 2 */
@@ -615,35 +659,56 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         for (int j = 0; j < TC; j++)
             done = done && sh.carry[j] == 0;
 
-        /* phase 4: colour and span */
+        /* phase 4: colour and span, CBATCH queue entries per thread
+         * at a time: their colour-quad gathers are all issued before the
+         * first colour is computed */
         const uint32_t nq = sh.qn;
-        for (uint32_t i = tid; i < nq; i += NT) {
-            uint32_t e = queue[i];
-            int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
-            int32_t row = (int32_t)(e & 0xFFFu);
-            float4 L = tl[skew(s)];
-            float fc = (float)(col0 + cc);
-            float x = L.x + (fc * L.z);
-            float y = L.y + (fc * L.w);
-            uint32_t colour;
-            if (MODE == kOutColour)
-                colour = color_bilinear_lut<CP2>(m, sh.sq, x, y);
-            else
-                colour = (floorf(x) == ceilf(x) || floorf(y) == ceilf(y))
-                             ? 0u : (uint32_t)(base + s + 1);
-            if (colour == 0u) {
-                sh.transparent = 1u;
-                continue;
+        for (uint32_t i0 = 0; i0 < nq; i0 += NT * CBATCH) {
+            uint32_t es[CBATCH];
+            float xs[CBATCH], ys[CBATCH];
+            uint4 cqs[CBATCH];
+#pragma unroll
+            for (int u = 0; u < CBATCH; u++) {
+                uint32_t e = queue[min(i0 + (uint32_t)(u * NT + tid), nq - 1u)];
+                int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
+                float4 L = tl[skew(s)];
+                float fc = (float)(col0 + cc);
+                es[u] = e;
+                xs[u] = L.x + (fc * L.z);
+                ys[u] = L.y + (fc * L.w);
+                if (MODE == kOutColour)
+                    cqs[u] = color_quad<CP2>(m, xs[u], ys[u]);
             }
-            int32_t bottom = (int32_t)qbot[i];
-            uint32_t *dst = stripe + cc * Hp;
-            int32_t stop = min(bottom, row + kShortSpan);
-            for (int32_t r = row; r < stop; r++)
-                dst[r] = colour;
-            if (bottom > stop) {
-                uint32_t li = atomicAdd(&sh.nlong, 1u);
-                longs[2 * li] = ((uint32_t)cc << 26) | ((uint32_t)stop << 13) | (uint32_t)bottom;
-                longs[2 * li + 1] = colour;
+#pragma unroll
+            for (int u = 0; u < CBATCH; u++) {
+                const uint32_t i = i0 + (uint32_t)(u * NT + tid);
+                if (i >= nq)
+                    break;
+                const uint32_t e = es[u];
+                int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
+                int32_t row = (int32_t)(e & 0xFFFu);
+                const float x = xs[u], y = ys[u];
+                uint32_t colour;
+                if (MODE == kOutColour)
+                    colour = color_of_quad(cqs[u], sh.sq, x, y);
+                else
+                    colour = (floorf(x) == ceilf(x) || floorf(y) == ceilf(y))
+                                 ? 0u : (uint32_t)(base + s + 1);
+                if (colour == 0u) {
+                    sh.transparent = 1u;
+                    continue;
+                }
+                int32_t bottom = (int32_t)qbot[i];
+                uint32_t *dst = stripe + cc * Hp;
+                int32_t stop = min(bottom, row + kShortSpan);
+                for (int32_t r = row; r < stop; r++)
+                    dst[r] = colour;
+                if (bottom > stop) {
+                    uint32_t li = atomicAdd(&sh.nlong, 1u);
+                    longs[2 * li] = ((uint32_t)cc << 26) | ((uint32_t)stop << 13) |
+                                    (uint32_t)bottom;
+                    longs[2 * li + 1] = colour;
+                }
             }
         }
         __syncthreads();
@@ -1047,8 +1112,24 @@ struct BatchArgs {
     const uint32_t *qmeta;
     uint32_t *counters;
     uint32_t *out;
+    const float4 *tab_lines;
+    const float *tab_iz;
     int32_t nqueues, chunk_max;
 };
+
+/* Slice records of every view of a batch, computed once when the batch is
+ * prepared: view v's slice k is entry views[v].tab + k. */
+__global__ void k_batch_tables(const fsk_batch_view *__restrict__ views,
+                               float4 *__restrict__ lines, float *__restrict__ iz)
+{
+    const fsk_batch_view &bv = views[blockIdx.y];
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < bv.f.n;
+         k += gridDim.x * blockDim.x) {
+        Slice sl = slice_params(bv.f, k);
+        lines[bv.tab + k] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
+        iz[bv.tab + k] = sl.iz;
+    }
+}
 
 /* XCD this wave runs on (hardware register XCC_ID). */
 __device__ __forceinline__ uint32_t xcc_id()
@@ -1097,8 +1178,11 @@ __global__ __launch_bounds__(NT, WPE) void k_render_batch(BatchArgs b)
         if (it == 0xFFFFFFFFu)
             break;
         const fsk_batch_view &bv = b.views[it >> 16];
-        render_tile<TC, NT, HBITS, HP2, CP2>(bv.m, bv.f, b.out + bv.offset, bv.f.cols,
-                                             it & 0xFFFFu, b.chunk_max, lds, sh);
+        /* shallower gather batches than the single-view kernel: the
+         * persistent loop's own state would otherwise spill registers */
+        render_tile<TC, NT, HBITS, HP2, CP2, kOutColour, 4, 1>(
+            bv.m, bv.f, b.out + bv.offset, bv.f.cols, it & 0xFFFFu, b.chunk_max, lds, sh,
+            b.tab_lines + bv.tab, b.tab_iz + bv.tab);
     }
 }
 
@@ -1127,6 +1211,8 @@ hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
     b.qmeta = fb->qmeta;
     b.counters = fb->counters;
     b.out = fb->out;
+    b.tab_lines = (const float4 *)fb->tab_lines;
+    b.tab_iz = fb->tab_iz;
     b.nqueues = fb->nqueues;
     b.chunk_max = chunk;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, fb->counters, fb->nqueues);
@@ -1544,6 +1630,18 @@ int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t)
 void fsk_set_tuning(const fsk_tuning *t)
 {
     g_tuning = *t;
+}
+
+int fsk_batch_tables(const fsk_batch *b, int32_t n_views, int32_t max_n, hipStream_t s)
+{
+    if (n_views <= 0 || max_n <= 0)
+        return hipSuccess;
+    int32_t bx = (max_n + 255) / 256;
+    if (bx > 64)
+        bx = 64;
+    hipLaunchKernelGGL(k_batch_tables, dim3(bx, n_views), dim3(256), 0, s, b->views,
+                       (float4 *)b->tab_lines, (float *)b->tab_iz);
+    return hipGetLastError();
 }
 
 int fsk_batch_run(const fsk_batch *b, hipStream_t s)
