@@ -1,8 +1,9 @@
 """Autotuner for the render kernels' shape.
 
 The kernels take three knobs (fsx_set_tuning): frame columns per
-workgroup tile (4 or 8), workgroups per CU for 4-column tiles (1 or 2,
-i.e. the register budget) and depth slices per chunk.  The tuner times
+workgroup tile (4 or 8), workgroups per CU (one of 1024 threads or two of
+512, which overlap each other's stalls but need shorter depth chunks to
+fit two tiles in LDS) and depth slices per chunk.  The tuner times
 every supported combination on the caller's own workload with the
 library's kernel events, keeps the fastest, and caches the choice in a
 JSON file keyed by device and workload signature.
@@ -12,6 +13,8 @@ import os
 import time
 
 CANDIDATES = (
+    (8, 2, 512),
+    (8, 2, 256),
     (8, 1, 2048),
     (8, 1, 1024),
     (8, 1, 512),

@@ -405,8 +405,8 @@ __device__ __forceinline__ uint32_t color_of_quad(uint4 q, const float *sq, floa
 }
 
 constexpr int kShortSpan = 24;   /* span rows a lane paints itself */
-constexpr int kGather = 8;       /* height gathers a lane keeps in flight */
-constexpr int kColourBatch = 4;  /* visible samples a lane colours per pass */
+constexpr int kGather = 2;       /* height gathers a lane keeps in flight */
+constexpr int kColourBatch = 1;  /* visible samples a lane colours per pass */
 
 /* Dynamic LDS of a tile render: slice table (float4 + inv_z per skewed
  * entry), the TC x rows colour stripe, the visible-sample queue and span
@@ -464,8 +464,8 @@ __device__ __forceinline__ uint32_t tile_value(uint32_t v, uint32_t sky)
 }
 
 /* chunk_max must be a multiple of NT/TC (the slice groups) and at most
- * kChunk; the LDS layout is sized for it. */
-template <int TC, int NT, int HBITS, bool HP2, bool CP2, int MODE = kOutColour,
+ * CMAX; the LDS layout is sized for it. */
+template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int MODE = kOutColour,
           int GATHER = kGather, int CBATCH = kColourBatch>
 __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f, uint32_t *out,
                                             int64_t pitch, int32_t tile, int32_t chunk_max,
@@ -475,8 +475,9 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 {
     constexpr int NW = NT / 64;                  /* waves */
     constexpr int SG = NT / TC;                  /* slice groups */
-    constexpr int KR = kChunk / SG;              /* slices per lane per chunk, at most */
-    static_assert(NW <= kMaxWaves && KR % 2 == 0 && KR <= 32, "tile shape");
+    constexpr int KR = CMAX / SG;                /* slices per lane per chunk, at most */
+    static_assert(NW <= kMaxWaves && KR % GATHER == 0 && GATHER % 2 == 0 && KR <= 32,
+                  "tile shape");
     const int32_t tabs = tab_stride(chunk_max);
     float4 *tl = (float4 *)lds;                                  /* [tabs] */
     float *tz = (float *)(lds + 4 * tabs);                       /* [tabs] */
@@ -832,15 +833,15 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     }
 }
 
-template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
+template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
 __global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
 {
     extern __shared__ uint32_t lds[];
     __shared__ TileShared<TC> sh;
     fill_square_lut(sh.sq, threadIdx.x);
-    render_tile<TC, NT, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
-                                               xcd_tile(blockIdx.x, a.ntiles), a.chunk_max, lds,
-                                               sh);
+    render_tile<TC, NT, CMAX, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
+                                                     xcd_tile(blockIdx.x, a.ntiles), a.chunk_max,
+                                                     lds, sh);
 }
 
 /* ------------------------------------------------------------------ */
@@ -960,10 +961,10 @@ __global__ void k_selftest(int which, unsigned long long *bad)
         atomicAdd(bad, local);
 }
 
-template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
+template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
 hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
 {
-    auto kern = k_render<TC, NT, HBITS, HP2, CP2, WPE, MODE>;
+    auto kern = k_render<TC, NT, CMAX, HBITS, HP2, CP2, WPE, MODE>;
     static int dyn_max = -1;
     if (dyn_max < 0) {
         /* the whole LDS of a CU minus the kernel's static shared memory */
@@ -994,10 +995,11 @@ hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
 /* ------------------------------------------------------------------ */
 
 /* Render kernel shape: columns per workgroup tile (8: 32-byte row
- * segments, one 1024-thread workgroup per CU; 4: 16-byte segments), for 4
- * columns one 1024-thread or two 512-thread workgroups per CU (the second
- * lets one tile's barrier and gather stalls overlap the other's work; it
- * needs the LDS of two tiles, so its depth chunk defaults to 1024), and the
+ * segments; 4: 16-byte segments), one 1024-thread or two 512-thread
+ * workgroups per CU (two let one tile's barrier and gather stalls overlap
+ * the other's work -- worth it when there are many more tiles than CUs, as
+ * in batches; they need the LDS of two tiles, so their depth chunk is
+ * shorter: at most 512 slices for 8 columns, 1024 for 4), and the
  * depth-chunk length.  fsk_set_tuning (the autotuner) or the
  * FSX_RENDER_TC / FSX_RENDER_WG_PER_CU / FSX_RENDER_CHUNK environment
  * variables choose them; 0 means automatic. */
@@ -1045,7 +1047,7 @@ inline int cu_count()
 }
 
 struct Shape {
-    int tc, nt, wg, chunk;
+    int tc, nt, wg, chunk, cmax;
 };
 
 constexpr size_t kStaticLds = 4096;   /* headroom for the kernels' static shared memory */
@@ -1053,9 +1055,18 @@ constexpr size_t kStaticLds = 4096;   /* headroom for the kernels' static shared
 inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
 {
     Shape s;
+    s.cmax = kChunk;
     int chunk = (t.chunk >= 256 && t.chunk <= kChunk) ? t.chunk : 0;
-    if (t.tc != 4 && render_lds_bytes_tc<8>(rows, chunk ? chunk : kChunk) + kStaticLds <=
-                         lds_per_cu()) {
+    int c8 = chunk && chunk <= 512 ? chunk : 512;
+    if (t.tc != 4 && t.wg_per_cu == 2 &&
+        2 * (render_lds_bytes_tc<8>(rows, c8) + kStaticLds) <= lds_per_cu()) {
+        s.tc = 8;
+        s.nt = 512;
+        s.wg = 2;
+        s.chunk = c8;
+        s.cmax = 512;
+    } else if (t.tc != 4 && render_lds_bytes_tc<8>(rows, chunk ? chunk : kChunk) + kStaticLds <=
+                                lds_per_cu()) {
         s.tc = 8;
         s.nt = 1024;
         s.wg = 1;
@@ -1095,11 +1106,12 @@ hipError_t launch_render(RenderArgs a, hipStream_t s)
     a.chunk_max = sh.chunk;
     if (sh.tc == 8) {
         a.ntiles = (a.f.cols + 7) / 8;
-        return launch_render_cfg<8, 1024, HBITS, HP2, CP2, 4, MODE>(a, s);
+        return sh.nt == 512 ? launch_render_cfg<8, 512, 512, HBITS, HP2, CP2, 4, MODE>(a, s)
+                            : launch_render_cfg<8, 1024, kChunk, HBITS, HP2, CP2, 4, MODE>(a, s);
     }
     a.ntiles = (a.f.cols + 3) / 4;
-    return sh.nt == 512 ? launch_render_cfg<4, 512, HBITS, HP2, CP2, 4, MODE>(a, s)
-                        : launch_render_cfg<4, 1024, HBITS, HP2, CP2, 4, MODE>(a, s);
+    return sh.nt == 512 ? launch_render_cfg<4, 512, kChunk, HBITS, HP2, CP2, 4, MODE>(a, s)
+                        : launch_render_cfg<4, 1024, kChunk, HBITS, HP2, CP2, 4, MODE>(a, s);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1149,7 +1161,7 @@ __global__ void k_batch_reset(uint32_t *counters, int32_t n)
  * neighbouring tiles of a view stay on one L2; a workgroup whose own queue
  * is drained takes from the next queues.  Every workgroup exits once all
  * queues report exhausted, after at most nqueues claims. */
-template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE>
+template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE>
 __global__ __launch_bounds__(NT, WPE) void k_render_batch(BatchArgs b)
 {
     extern __shared__ uint32_t lds[];
@@ -1180,16 +1192,17 @@ __global__ __launch_bounds__(NT, WPE) void k_render_batch(BatchArgs b)
         const fsk_batch_view &bv = b.views[it >> 16];
         /* shallower gather batches than the single-view kernel: the
          * persistent loop's own state would otherwise spill registers */
-        render_tile<TC, NT, HBITS, HP2, CP2, kOutColour, 4, 1>(
+        render_tile<TC, NT, CMAX, HBITS, HP2, CP2, kOutColour, (CMAX / (NT / TC) < 4 ? 2 : 4),
+                    1>(
             bv.m, bv.f, b.out + bv.offset, bv.f.cols, it & 0xFFFFu, b.chunk_max, lds, sh,
             b.tab_lines + bv.tab, b.tab_iz + bv.tab);
     }
 }
 
-template <int TC, int NT, int HBITS, bool HP2, bool CP2, int WPE>
+template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE>
 hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
 {
-    auto kern = k_render_batch<TC, NT, HBITS, HP2, CP2, WPE>;
+    auto kern = k_render_batch<TC, NT, CMAX, HBITS, HP2, CP2, WPE>;
     static int dyn_max = -1;
     if (dyn_max < 0) {
         hipFuncAttributes fa;
@@ -1225,9 +1238,10 @@ hipError_t launch_batch(const fsk_batch *fb, hipStream_t s)
 {
     Shape sh = resolve_shape(fb->max_rows, fb->tune);
     if (sh.tc == 8)
-        return launch_batch_cfg<8, 1024, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
-    return sh.nt == 512 ? launch_batch_cfg<4, 512, HBITS, HP2, CP2, 4>(fb, sh.chunk, s)
-                        : launch_batch_cfg<4, 1024, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
+        return sh.nt == 512 ? launch_batch_cfg<8, 512, 512, HBITS, HP2, CP2, 4>(fb, sh.chunk, s)
+                            : launch_batch_cfg<8, 1024, kChunk, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
+    return sh.nt == 512 ? launch_batch_cfg<4, 512, kChunk, HBITS, HP2, CP2, 4>(fb, sh.chunk, s)
+                        : launch_batch_cfg<4, 1024, kChunk, HBITS, HP2, CP2, 4>(fb, sh.chunk, s);
 }
 
 /* ------------------------------------------------------------------ */

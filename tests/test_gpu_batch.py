@@ -56,8 +56,8 @@ def pow2_maps(ctx):
     return [ctx.map(c, h) for c, h in data], [h.shape for _, h in data]
 
 
-@pytest.mark.parametrize("shape", [(0, 0, 0), (8, 1, 2048), (8, 1, 512), (4, 2, 2048),
-                                   (4, 1, 1024)])
+@pytest.mark.parametrize("shape", [(0, 0, 0), (8, 1, 2048), (8, 1, 512), (8, 2, 512),
+                                   (8, 2, 256), (4, 2, 1024), (4, 1, 1024)])
 def test_batch_matches_single_views(ctx, pow2_maps, shape):
     maps, shapes = pow2_maps
     ctx.set_tuning(*shape)

@@ -183,3 +183,25 @@ def test_reference_map_data_bitexact(ctx):
         want = O.render_off(v, shaded, alt)
         assert (want == v.sky_color).any() and (want != v.sky_color).any()
         _check(ctx.render(m, v), want)
+
+
+@pytest.mark.parametrize("shape", [(8, 2, 512), (8, 2, 256), (8, 1, 512), (4, 2, 1024),
+                                   (4, 1, 2048)])
+def test_kernel_shapes_bitexact(ctx, small_scene, shape):
+    """Every tile shape the autotuner may pick renders the same frames."""
+    col, hts = small_scene
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    ctx.set_tuning(*shape)
+    try:
+        for vi in (0, 1, 3, 6):
+            x, y, hh, a, hor, dist, fov, h, w, d, z0, sc, n = SMALL_VIEWS[vi]
+            v = View(x=x, y=y, height=hh, angle=a, horizon=hor, distance=dist, fov=fov,
+                     sky_color=sky_color(0.1), h=h, w=w, d=d, z0=z0, scale=sc, n_slices=n)
+            _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+        v = View(x=70.0, y=30.0, height=400.0, angle=0.9, horizon=-50.0, distance=600.0,
+                 fov=1.1, sky_color=sky_color(0.1), h=300, w=61, d=0.0005, z0=0.2,
+                 n_slices=2600)
+        _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+    finally:
+        ctx.set_tuning(0, 0, 0)
