@@ -680,6 +680,20 @@ static struct fs_terrain *terrain_new(struct futhark_context *ctx, const uint32_
  * sun_ang): generate_shadowmap_accumulated (fut/effects.fut:108-125) with
  * sun = R_y(sun_ang) R_z(sun_height) [0,1,0] (fut/interactive.fut:196),
  * whose closed form is (cos a * -sin h, cos h, -sin a * -sin h). */
+/* vec3_rotate #y sun_ang (vec3_rotate #z sun_height [0, 1, 0])
+ * (fut/effects.fut:6-25): matvecmul rows folded left from 0; the zero
+ * terms do not change the non-zero sums, so the rotations reduce to
+ * (cos a * (-sin h), cos h, (-sin a) * (-sin h)) bit for bit. */
+static void sun_vector(float sun_height, float sun_ang, float out[3])
+{
+    float sh = sinf(sun_height), chh = cosf(sun_height);
+    float sa = sinf(sun_ang), ca = cosf(sun_ang);
+    float v0 = -sh, v1 = chh;
+    out[0] = ca * v0;
+    out[1] = v1;
+    out[2] = (-sa) * v0;
+}
+
 static struct fs_shade *shade_new(struct futhark_context *ctx, const struct fs_terrain *t,
                                   int32_t size, float sun_height, float sun_ang)
 {
@@ -698,19 +712,9 @@ static struct fs_shade *shade_new(struct futhark_context *ctx, const struct fs_t
         hipFree(s->d_shaded); hipFree(s->d_cq); free(s);
         return NULL;
     }
-    float sh = sinf(sun_height), chh = cosf(sun_height);
-    float sa = sinf(sun_ang), ca = cosf(sun_ang);
-    /* matvecmul rows folded left from 0; the zero terms do not change the
-     * non-zero sums */
-    float v0 = -sh, v1 = chh;
-    float sun0 = ca * v0;
-    float sun1 = v1;
-    float sun2 = (-sa) * v0;
-    s->sun[0] = sun0;
-    s->sun[1] = sun1;
-    s->sun[2] = sun2;
+    sun_vector(sun_height, sun_ang, s->sun);
     e = (hipError_t)fsk_shade(t->d_hp, t->rows, t->cols, t->d_color, t->rows, t->cols,
-                              sun0, sun1, sun2, size, s->d_shaded, ctx->stream);
+                              s->sun[0], s->sun[1], s->sun[2], size, s->d_shaded, ctx->stream);
     if (e == hipSuccess)
         e = (hipError_t)fsk_color_quads(s->d_shaded, size, size, s->d_cq, ctx->stream);
     if (e != hipSuccess) {
@@ -1022,6 +1026,58 @@ static uint32_t u8_of(float v)
     return (uint32_t)(uint64_t)(int64_t)s;
 }
 
+/* argb.mix with the Gloss rule: normalised weights, channels as the
+ * root of the weighted squares, alpha mixed linearly. */
+static uint32_t argb_mix(float m1, uint32_t c1, float m2, uint32_t c2)
+{
+    float p[4], q[4];
+    for (int k = 0; k < 4; k++) {
+        int sh = k == 3 ? 24 : 16 - 8 * k;
+        p[k] = (float)((c1 >> sh) & 255u) / 255.0f;
+        q[k] = (float)((c2 >> sh) & 255u) / 255.0f;
+    }
+    float m12 = m1 + m2;
+    float m1n = m1 / m12, m2n = m2 / m12;
+    uint32_t out = u8_of(((m1 * p[3]) + (m2 * q[3])) / m12) << 24;
+    for (int k = 0; k < 3; k++)
+        out |= u8_of(sqrtf((m1n * (p[k] * p[k])) + (m2n * (q[k] * q[k])))) << (16 - 8 * k);
+    return out;
+}
+
+/* expo2, fut/render_functions.fut:23-28 (heights of #math mode). */
+static float expo2(float x, float y)
+{
+    float output = (powf(x, 2.0f) - powf(y, 2.0f)) / 1024.0f;
+    return (x > -1024.0f && x < 1024.0f && y > -1024.0f && y < 1024.0f) ? output : 0.0f;
+}
+
+/* function_coloring, fut/render_functions.fut:37-41 (colours of #math mode). */
+static uint32_t function_coloring(float x, float y)
+{
+    return (mod_i32(i32_f32(x), 64) == 0 || mod_i32(i32_f32(y), 64) == 0) ? 0xFF000000u
+                                                                           : 0xFFFFFFFFu;
+}
+
+/* The texel png_color reads at (x, y) from the #math shadow map
+ * generate_shadowmap_accumulated function_coloring expo2 sun
+ * (fut/interactive.fut:176, fut/effects.fut:108-125): only the one texel
+ * is evaluated, with the map's own loop. */
+static uint32_t math_texel(float x, float y, const float sun[3])
+{
+    int32_t tx = mod_i32(i32_f32(x), 1024), ty = mod_i32(i32_f32(y), 1024);
+    const float step_size = 4.0f;
+    float fx = (float)tx, fy = (float)ty;
+    float height = expo2(fx, fy);
+    int32_t count = 0;
+    for (int32_t dist = 1; dist < 256; dist++) {
+        float t = (float)dist * step_size;
+        float probe = expo2(fx + (t * sun[0]), fy + (t * sun[2]));
+        if (((height + (t * sun[1])) - probe) < -0.5f)
+            count += 1;
+    }
+    return argb_mix(step_size * (float)count, 0xFF000000u, 1.0f, function_coloring(fx, fy));
+}
+
 static uint32_t argb_scale(uint32_t col, float s)
 {
     float r = (float)((col >> 16) & 255u) / 255.0f;
@@ -1166,20 +1222,24 @@ int futhark_entry_update_map(struct futhark_context *ctx, struct futhark_opaque_
 }
 
 /* render, fut/interactive.fut:173-183 -> voxel_renderer.render with
- * z_0 = 0, d = 0.001 and scale f32(w/2) (fut/voxel_renderer.fut:101-251). */
+ * z_0 = 0, d = 0.001 and scale f32(w/2) (fut/voxel_renderer.fut:101-251).
+ *
+ * render fixes z_0 = 0 for every mode, so its first depth slice lies at
+ * z = 0: inv_z = (1/0) * f32(w/2) is +inf (NaN for w = 1), every column's
+ * first sample sits at (cam.x, cam.y) and projects, through the x86 cast of
+ * +-inf or NaN, to row 0; no later slice can rise above row 0, so each
+ * column shows that one sample on every row -- or the sky if its colour is
+ * 0.  The #on smoothing branch (:174-213) reaches the same frame: every
+ * slice's running minimum is slice 0's, so the entry that survives the
+ * scatter pairs slice 0 with itself and is not blended.  #png frames are
+ * rendered by the HIP kernel like any other view (which also covers the
+ * general case); a #math frame (:175-178) is the colour of the #math
+ * shadow map under the camera, filled on the device. */
 int futhark_entry_render(struct futhark_context *ctx, struct futhark_u32_2d **out0,
                          const struct futhark_opaque_state *in0)
 {
     if (!ctx || !out0 || !in0)
         return 1;
-    if (in0->mode != MODE_PNG) {
-        set_error(ctx, "render: #math mode is not implemented by this renderer");
-        return 1;
-    }
-    if (in0->smoothing != SMOOTH_OFF) {
-        set_error(ctx, "render: smoothing #on is not implemented by this renderer");
-        return 1;
-    }
     if (in0->height < 0 || in0->width < 0) {
         set_error(ctx, "render: negative frame size");
         return 1;
@@ -1210,7 +1270,30 @@ int futhark_entry_render(struct futhark_context *ctx, struct futhark_u32_2d **ou
         free(a);
         return 1;
     }
-    if (render_into(ctx, in0->terrain, in0->shade, &v, a->d, v.w)) {
+    if (in0->mode == MODE_MATH) {
+        size_t npx = (size_t)v.h * (size_t)v.w;
+        uint32_t c = 0;
+        if (npx > 0 && zs_count(v.distance, v.d, v.z0) > 0) {
+            /* the first slice's sample of column 0, as get_h_line and
+             * get_segment form it at z = 0 (every column shares it) */
+            fsk_frame f;
+            frame_of_view(&v, &f);
+            const float z = 0.0f;
+            float lx = f.Lx * z, ly = f.Ly * z, rx = f.Rx * z, ry = f.Ry * z;
+            float dx = (rx - lx) / f.fw, dy = (ry - ly) / f.fw;
+            float x0 = (lx + f.cam_x) + (0.0f * dx);
+            float y0 = (ly + f.cam_y) + (0.0f * dy);
+            float sun[3];
+            sun_vector(in0->sun_height, in0->sun_ang, sun);
+            c = math_texel(x0, y0, sun);
+        }
+        hipError_t e = (hipError_t)fsk_fill_u32(a->d, npx, c ? c : v.sky_color, ctx->stream);
+        if (e != hipSuccess) {
+            set_error(ctx, "render: %s", hipGetErrorString(e));
+            futhark_free_u32_2d(ctx, a);
+            return 1;
+        }
+    } else if (render_into(ctx, in0->terrain, in0->shade, &v, a->d, v.w)) {
         futhark_free_u32_2d(ctx, a);
         return 1;
     }
@@ -1336,6 +1419,59 @@ int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,
     }
     dev_release(ctx, d, bytes);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* The stateless program of tests/futspace.fut                          */
+/* ------------------------------------------------------------------ */
+
+/* get_zs delta cam.distance z_0 with delta = 0.005, z_0 = 1
+ * (tests/futspace.fut:47-54, :84-86): n = i32.f32 (num / div). */
+static int32_t stateless_zs_count(float d)
+{
+    const float delta = 0.005f, z_0 = 1.0f;
+    float sq = sqrtf(powf(delta - 2.0f * z_0, 2.0f) + (8.0f * delta) * d);
+    float num = (sq - 2.0f * z_0) + delta;
+    return i32_f32(num / (2.0f * delta));
+}
+
+int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
+                         const int32_t *dev_height, int32_t q, int32_t r,
+                         const struct fsx_view *view, int32_t *dev_out)
+{
+    if (!ctx || !view)
+        return 1;
+    if (view->h < 0 || view->w < 0 || view->h > fsk_render_max_rows() || q <= 0 || r <= 0 ||
+        (int64_t)q * r > ((int64_t)1 << 30)) {
+        set_error(ctx, "fsx_render_stateless: unsupported sizes (frame %d x %d, map %d x %d)",
+                  view->h, view->w, q, r);
+        return 1;
+    }
+    if (view->h == 0 || view->w == 0)
+        return 0;
+    if (!dev_color || !dev_height || !dev_out) {
+        set_error(ctx, "fsx_render_stateless: null buffer");
+        return 1;
+    }
+    fsk_frame f;
+    frame_of_view(view, &f);
+    f.d = 0.005f;
+    f.z0 = 1.0f;
+    f.scale = 240.0f;
+    f.n = stateless_zs_count(view->distance);
+    fsk_map m;
+    m.hq = dev_height;
+    m.hrows = q;
+    m.hcols = r;
+    m.hbits = 32;
+    m.cq = dev_color;
+    m.crows = q;
+    m.ccols = r;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int pe = prof_begin(ctx);
+    HIPCHECK(ctx, fsk_render_nearest(&m, &f, (uint32_t *)dev_out, view->w, ctx->stream));
+    prof_end(ctx, pe);
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -107,12 +107,18 @@ int fsk_color_quads(const uint32_t *d_col, int32_t rows, int32_t cols, void *d_c
  * *d_gsky (both accumulate). */
 int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
                       hipStream_t s);
+/* The stateless program of tests/futspace.fut: m.hq = i32 heights and
+ * m.cq = i32 colours, both [hrows][hcols], nearest sampling; f.sky is the
+ * program's sky (also its `empty` colour). */
+int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
+                       hipStream_t s);
 int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                    int32_t wc, float sky_scale, const int32_t *d_src, float *d_img, hipStream_t s);
 int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                   int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
                   float *d_gsky, hipStream_t s);
 int fsk_zero_f32(float *d_p, size_t n, hipStream_t s);
+int fsk_fill_u32(uint32_t *d_p, size_t n, uint32_t v, hipStream_t s);
 int fsk_l2_loss(const float *d_img, const float *d_tgt, size_t n, float *d_g, float *d_loss,
                 hipStream_t s);
 int fsk_shade_factor(const uint16_t *d_hplain, int32_t hrows, int32_t hcols, float sun0,

@@ -453,14 +453,33 @@ struct TileShared {
  * visibility and fill rules; a sample is transparent there iff its map
  * coordinate is integral in x or y, which is where argb.mix's weight sum
  * vanishes. */
-enum { kOutColour = 0, kOutSlice = 1 };
+enum { kOutColour = 0, kOutSlice = 1, kOutNearest = 2 };
 
+/* kOutNearest is the stateless program of tests/futspace.fut:47-145: the
+ * map is plain i32 colours (m.cq) and unmasked i32 heights (m.hq), sampled
+ * at the truncated coordinate (floor-mod wrapped), and the column starts
+ * as sky with sky as the `empty` colour of its fill (:110-121) -- colour 0
+ * is an ordinary colour there.  The stripe holds colour ^ sky, so that its
+ * 0 (unpainted) is the sky and a sky-coloured sample is the transparent
+ * one. */
 template <int MODE>
 __device__ __forceinline__ uint32_t tile_value(uint32_t v, uint32_t sky)
 {
     if (MODE == kOutColour)
         return v ? v : sky;
+    if (MODE == kOutNearest)
+        return v ^ sky;
     return v ? v - 1u : 0xFFFFFFFFu;
+}
+
+/* Texel index of the stateless program's sample at (x, y): i32.f32 of
+ * each coordinate (x86 truncation), floor-mod by the map size. */
+template <bool P2>
+__device__ __forceinline__ uint32_t nearest_index(const fsk_map &m, float x, float y)
+{
+    int32_t ix = wrap<P2>(i32_f32_x86(x), m.hcols);
+    int32_t iy = wrap<P2>(i32_f32_x86(y), m.hrows);
+    return (uint32_t)iy * (uint32_t)m.hcols + (uint32_t)ix;
 }
 
 /* chunk_max must be a multiple of NT/TC (the slice groups) and at most
@@ -555,12 +574,17 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 xs[u] = L.x + (fcol * L.z);
                 ys[u] = L.y + (fcol * L.w);
                 izs[u] = tz[o];
-                qs[u] = height_quad<HBITS, HP2>(m, xs[u], ys[u]);
+                if (MODE == kOutNearest)
+                    qs[u] = make_uint2(
+                        ((const uint32_t *)m.hq)[nearest_index<HP2>(m, xs[u], ys[u])], 0u);
+                else
+                    qs[u] = height_quad<HBITS, HP2>(m, xs[u], ys[u]);
             }
 #pragma unroll
             for (int u = 0; u < GATHER; u++) {
                 const int j = j0 + u;
-                float hgt = height_interp<HBITS>(qs[u], xs[u], ys[u]);
+                float hgt = MODE == kOutNearest ? (float)(int32_t)qs[u].x
+                                                : height_interp<HBITS>(qs[u], xs[u], ys[u]);
                 float rel = ((f.cam_h - hgt) * izs[u]) + f.horizon;
                 /* rows are at most H: an invalid sample is raised to H
                  * arithmetically, so that no branch splits the group */
@@ -692,6 +716,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 uint32_t colour;
                 if (MODE == kOutColour)
                     colour = color_of_quad(cqs[u], sh.sq, x, y);
+                else if (MODE == kOutNearest)
+                    colour = ((const uint32_t *)m.cq)[nearest_index<HP2>(m, x, y)] ^ f.sky;
                 else
                     colour = (floorf(x) == ceilf(x) || floorf(y) == ceilf(y))
                                  ? 0u : (uint32_t)(base + s + 1);
@@ -1581,6 +1607,14 @@ __global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
         unsafeAtomicAdd(loss, (part[0] + part[1]) + (part[2] + part[3]));
 }
 
+/* A frame of one colour (the degenerate frames of the state's #math mode). */
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
 /* Zero fill (a kernel, not a memset node, so that callers may capture it). */
 __global__ void k_zero_f32(float *__restrict__ p, size_t n)
 {
@@ -1709,6 +1743,26 @@ static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, 
     return a;
 }
 
+int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
+                       hipStream_t s)
+{
+    if (f->rows <= 0 || f->cols <= 0)
+        return hipSuccess;
+    if (f->rows > kMaxRows)
+        return hipErrorInvalidValue;
+    RenderArgs a;
+    a.m = *m;
+    a.f = *f;
+    if (a.f.n < 0)
+        a.f.n = 0;
+    a.out = d_out;
+    a.pitch = pitch;
+    a.ntiles = 0;
+    a.chunk_max = kChunk;
+    return pow2(m->hrows) && pow2(m->hcols) ? launch_render<8, true, false, kOutNearest>(a, s)
+                                            : launch_render<8, false, false, kOutNearest>(a, s);
+}
+
 int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                    int32_t wc, float sky_scale, const int32_t *d_src, float *d_img, hipStream_t s)
 {
@@ -1734,6 +1788,17 @@ int fsk_l2_loss(const float *d_img, const float *d_tgt, size_t n, float *d_g, fl
         blocks = 1;
     hipLaunchKernelGGL(k_l2_loss, dim3((unsigned)blocks), dim3(256), 0, s, d_img, d_tgt, n, d_g,
                        d_loss);
+    return hipGetLastError();
+}
+
+int fsk_fill_u32(uint32_t *d_p, size_t n, uint32_t v, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, d_p, n, v);
     return hipGetLastError();
 }
 

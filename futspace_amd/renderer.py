@@ -50,6 +50,12 @@ class View:
                        self.scale, self.n_slices, self.map, 0)
 
 
+# `main` of tests/futspace.fut:127-145 (its depth fields are fixed by the
+# program: z_0 = 1, delta = 0.005, scale 240)
+STATELESS_MAIN = View(x=512.0, y=800.0, height=78.0, angle=0.0, horizon=100.0, distance=800.0,
+                      fov=1.0, sky_color=0xFF9090E0, h=400, w=800)
+
+
 def sky_color(sun_height: float) -> int:
     """argb.scale 0xFF9090e0 sun_height, as `step` sets it (fut/interactive.fut:163)."""
     return int(_native.futspace().fsx_sky(ctypes.c_float(sun_height)))
@@ -142,6 +148,30 @@ class Context:
         """A prepared batch that can be run, captured into a HIP graph and
         replayed (fsx_batch_*)."""
         return Batch(self, maps, views, out, offsets)
+
+    def render_stateless(self, color, height, view: View = None):
+        """The stateless program of tests/futspace.fut (fsx_render_stateless):
+        i32 colour and height maps [q][r] (numpy or CUDA tensors); view=None
+        is the program's `main` (camera (512, 800, 78, 0, 100, 800, fov 1),
+        sky 0xFF9090e0, 400 x 800).  Returns the [h][w] int32 frame."""
+        import torch
+        if view is None:
+            view = STATELESS_MAIN
+        c = torch.as_tensor(np.ascontiguousarray(color, dtype=np.int32)).cuda() \
+            if not isinstance(color, torch.Tensor) else color.contiguous()
+        h = torch.as_tensor(np.ascontiguousarray(height, dtype=np.int32)).cuda() \
+            if not isinstance(height, torch.Tensor) else height.contiguous()
+        if c.shape != h.shape or c.dim() != 2:
+            raise ValueError("colour and height maps must be [q][r] arrays of one shape")
+        out = torch.empty((view.h, view.w), dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        cv = view.to_c()
+        self.check(self.lib.fsx_render_stateless(
+            self.ptr, ctypes.c_void_p(c.data_ptr()), ctypes.c_void_p(h.data_ptr()),
+            c.shape[0], c.shape[1], ctypes.byref(cv), ctypes.c_void_p(out.data_ptr())),
+            "fsx_render_stateless")
+        self.sync()
+        return out.cpu().numpy()
 
     def set_tuning(self, tile_columns=0, wg_per_cu=0, chunk=0):
         """Render kernel shape (0 = automatic); see fsx_set_tuning."""

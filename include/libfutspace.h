@@ -180,6 +180,19 @@ int fsx_batch_capture(struct futhark_context *ctx, struct fsx_batch *batch);
 int fsx_batch_replay(struct futhark_context *ctx, struct fsx_batch *batch);
 int fsx_batch_free(struct futhark_context *ctx, struct fsx_batch *batch);
 
+/* The stateless program of tests/futspace.fut:47-145 (`render`): nearest
+ * sampling (i32.f32 truncation, floor-mod) of i32 colours and unmasked i32
+ * heights, both [q][r] device arrays; z_0 = 1, delta = 0.005, inv_z =
+ * (1/z) * 240 as the program fixes them; view supplies the camera
+ * (x, y, height, angle, horizon, distance, fov), the frame size (l = h,
+ * m = w) and the sky colour, which is also the fill's `empty` colour.
+ * Writes dev_out[h][w] i32, asynchronously on the context stream.  The
+ * program's `main` is this with camera (512, 800, 78, 0, 100, 800, fov 1),
+ * sky 0xFF9090e0 and a 400 x 800 frame (:127-145). */
+int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
+                         const int32_t *dev_height, int32_t q, int32_t r,
+                         const struct fsx_view *view, int32_t *dev_out);
+
 /* Differentiable renderer: the f32 relaxation of the png render path
  * (fut/interactive.fut:179-182 over fut/voxel_renderer.fut:214-251 with
  * png_color_filtered, fut/render_functions.fut:95-105), with the shadow

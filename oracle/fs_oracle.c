@@ -532,7 +532,215 @@ int fso_shadowmap(const uint32_t *col, int32_t hc, int32_t wc,
     return 0;
 }
 
-/* Sky colour set by `step`, fut/interactive.fut:163. */
+/* ------------------------------------------------------------------ */
+/* #math mode samplers, fut/render_functions.fut:23-42                  */
+/* ------------------------------------------------------------------ */
+
+/* expo2: (x**2 - y**2) / 1024 inside (-1024, 1024)^2, else 0 (`**` is
+ * powf, as futhark c emits it). */
+float fso_expo2(float x, float y)
+{
+    float output = (powf(x, 2.0f) - powf(y, 2.0f)) / 1024.0f;
+    if (x > -1024.0f && x < 1024.0f && y > -1024.0f && y < 1024.0f)
+        return output;
+    return 0.0f;
+}
+
+/* function_coloring: black on the 64-texel grid lines, else white. */
+uint32_t fso_function_coloring(float x, float y)
+{
+    if (fso_mod_i32(fso_i32_f32(x), 64) == 0 || fso_mod_i32(fso_i32_f32(y), 64) == 0)
+        return fso_black();
+    return fso_from_rgba(1.0f, 1.0f, 1.0f, 1.0f);
+}
+
+/* generate_shadowmap_accumulated function_coloring expo2 sun
+ * (fut/effects.fut:108-125 with the #math samplers), 1024^2. */
+int fso_math_shadowmap(const float sun[3], uint32_t *out)
+{
+    const float step_size = (float)(SHADOW_MAX_DIST / SHADOW_STEPS);
+    const uint32_t black = fso_black();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int32_t y = 0; y < 1024; y++) {
+        for (int32_t x = 0; x < 1024; x++) {
+            float fx = (float)x, fy = (float)y;
+            float height = fso_expo2(fx, fy);
+            int32_t count = 0;
+            for (int32_t dist = 1; dist < SHADOW_STEPS; dist++) {
+                float t = (float)dist * step_size;
+                float probe = fso_expo2(fx + (t * sun[0]), fy + (t * sun[2]));
+                if (((height + (t * sun[1])) - probe) < -0.5f)
+                    count += 1;
+            }
+            out[(size_t)y * 1024 + x] =
+                fso_mix(step_size * (float)count, black, 1.0f, fso_function_coloring(fx, fy));
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* The state's render, fut/interactive.fut:173-183 over                 */
+/* fut/voxel_renderer.fut:101-251, both smoothing branches              */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t mode;                 /* 0 #math, 1 #png */
+    const uint32_t *col;          /* #png: shadowed colours; #math: the math shadow map */
+    int32_t hc, wc;
+    const int32_t *alt;
+    int32_t ha, wa;
+} state_samplers;
+
+static uint32_t state_colour(const state_samplers *sm, float x, float y)
+{
+    if (sm->mode == 1)
+        return fso_png_color_filtered(sm->col, sm->hc, sm->wc, x, y);
+    return fso_png_color(sm->col, 1024, 1024, x, y);
+}
+
+static float state_height(const state_samplers *sm, float x, float y)
+{
+    if (sm->mode == 1)
+        return fso_png_height_filtered(sm->alt, sm->ha, sm->wa, x, y);
+    return fso_expo2(x, y);
+}
+
+typedef struct {
+    uint32_t c0, c1;
+    int32_t h0, h1, i0, i1;
+} smooth_entry;
+
+static int entry_is_neutral(const smooth_entry *e)
+{
+    return e->c0 == 0 && e->c1 == 0 && e->h0 == 0 && e->h1 == 0 && e->i0 == 1000 &&
+           e->i1 == 1000;
+}
+
+/* render with z_0 = 0, d = 0.001 and inv_z = (1/z) * f32(w/2)
+ * (fut/voxel_renderer.fut:101-104, :217).  #off: scan occlude, scatter,
+ * scan fill_vline, sky (:214-251).  #on (:174-213): scan occlude2 over
+ * (colour, row, slice), the tuple of each slice with its predecessor's
+ * (rotate (-1), so slice 0 pairs with the last slice), scatter into a
+ * column of (0,0,0,0,1000,1000) -- the last write to a row wins, as in
+ * futhark c's sequential scatter -- scan fill_vline3, then rows whose
+ * entry pairs slice i with slice i-1 are argb.mix-ed between the two
+ * colours across the span; finally 0 -> sky. */
+int fso_render_state(const fso_camera *cam, int32_t mode, int32_t smoothing_on,
+                     const uint32_t *shaded, int32_t hc, int32_t wc,
+                     const int32_t *alt, int32_t ha, int32_t wa, const float sun[3],
+                     int32_t h, int32_t w, uint32_t *out)
+{
+    if (h < 0 || w < 0)
+        return 1;
+    state_samplers sm = {mode, shaded, hc, wc, alt, ha, wa};
+    uint32_t *mathmap = NULL;
+    if (mode == 0) {
+        mathmap = (uint32_t *)malloc(sizeof(uint32_t) * 1024 * 1024);
+        if (!mathmap)
+            return 1;
+        fso_math_shadowmap(sun, mathmap);
+        sm.col = mathmap;
+    }
+    const float d = 0.001f, z_0 = 0.0f;
+    int32_t n = fso_get_zs_count(cam->distance, d, z_0);
+    if (n < 0)
+        n = 0;
+    float *zs = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    uint32_t *pc = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1) * (size_t)(w > 0 ? w : 1));
+    int32_t *ph = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1) * (size_t)(w > 0 ? w : 1));
+    smooth_entry *vl = (smooth_entry *)malloc(sizeof(smooth_entry) * (size_t)(h > 0 ? h : 1));
+    smooth_entry *cx = (smooth_entry *)malloc(sizeof(smooth_entry) * (size_t)(n > 0 ? n : 1));
+    uint32_t *colors = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t *heights = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t *idxs = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int rc = 1;
+    if (!zs || !pc || !ph || !vl || !cx || !colors || !heights || !idxs)
+        goto done;
+    fso_get_zs(cam->distance, d, z_0, zs, n);
+    for (int32_t k = 0; k < n; k++) {
+        float line[4];
+        fso_get_h_line(zs[k], cam, w, line);
+        float inv_z = (1.0f / zs[k]) * (float)(w / 2);
+        for (int32_t i = 0; i < w; i++) {
+            float x = line[0] + ((float)i * line[2]);
+            float y = line[1] + ((float)i * line[3]);
+            pc[(size_t)k * w + i] = state_colour(&sm, x, y);
+            ph[(size_t)k * w + i] = project(cam, state_height(&sm, x, y), inv_z);
+        }
+    }
+    for (int32_t i = 0; i < w; i++) {
+        if (!smoothing_on) {
+            uint32_t *vline = (uint32_t *)vl;       /* h u32 fit in h entries */
+            for (int32_t r = 0; r < h; r++)
+                vline[r] = 0;
+            uint32_t acc_c = 0;
+            int32_t acc_h = h;
+            for (int32_t k = 0; k < n; k++) {
+                uint32_t c2 = pc[(size_t)k * w + i];
+                int32_t h2 = ph[(size_t)k * w + i];
+                if (!(acc_h <= h2)) {
+                    acc_c = c2;
+                    acc_h = h2;
+                }
+                if (acc_h >= 0 && acc_h < h)
+                    vline[acc_h] = acc_c;
+            }
+            uint32_t fill = 0;
+            for (int32_t r = 0; r < h; r++) {
+                if (vline[r] != 0)
+                    fill = vline[r];
+                out[(size_t)r * w + i] = fill == 0 ? cam->sky_color : fill;
+            }
+            continue;
+        }
+        uint32_t acc_c = 0;
+        int32_t acc_h = h, acc_i = 0;
+        for (int32_t k = 0; k < n; k++) {
+            uint32_t c2 = pc[(size_t)k * w + i];
+            int32_t h2 = ph[(size_t)k * w + i];
+            if (!(acc_h <= h2)) {
+                acc_c = c2;
+                acc_h = h2;
+                acc_i = k;
+            }
+            colors[k] = acc_c;
+            heights[k] = acc_h;
+            idxs[k] = acc_i;
+        }
+        for (int32_t k = 0; k < n; k++) {
+            int32_t p = (k - 1 + n) % n;
+            smooth_entry e = {colors[k], colors[p], heights[k], heights[p], idxs[k], idxs[p]};
+            cx[k] = e;
+        }
+        const smooth_entry neutral = {0, 0, 0, 0, 1000, 1000};
+        for (int32_t r = 0; r < h; r++)
+            vl[r] = neutral;
+        for (int32_t k = 0; k < n; k++)
+            if (heights[k] >= 0 && heights[k] < h)
+                vl[heights[k]] = cx[k];
+        smooth_entry acc = neutral;
+        for (int32_t r = 0; r < h; r++) {
+            if (!entry_is_neutral(&vl[r]))
+                acc = vl[r];
+            float range = fmaxf(1.0f, (float)(acc.h1 - acc.h0));
+            float delta1 = fabsf((float)r - (float)acc.h1) / range;
+            float delta2 = fabsf((float)acc.h0 - (float)r) / range;
+            uint32_t colour = (acc.i0 - acc.i1 == 1) ? fso_mix(delta2, acc.c1, delta1, acc.c0)
+                                                      : acc.c0;
+            out[(size_t)r * w + i] = colour == 0 ? cam->sky_color : colour;
+        }
+    }
+    rc = 0;
+done:
+    free(zs); free(pc); free(ph); free(vl); free(cx); free(colors); free(heights); free(idxs);
+    free(mathmap);
+    return rc;
+}
+
+/* Sky colour set by `step`, fut/interactive.fut:163. *//* Sky colour set by `step`, fut/interactive.fut:163. */
 uint32_t fso_sky(float sun_height)
 {
     return fso_scale(0xFF9090E0u, sun_height);

@@ -124,6 +124,17 @@ int  fso_shadow_counts(const int32_t *alt, int32_t ha, int32_t wa, const float s
                        int32_t size, int32_t *counts);
 uint32_t fso_sky(float sun_height);
 
+/* ---- #math samplers and the state's render (both smoothing modes) ---- */
+float    fso_expo2(float x, float y);
+uint32_t fso_function_coloring(float x, float y);
+int      fso_math_shadowmap(const float sun[3], uint32_t *out);   /* [1024][1024] */
+/* The state's render entry: mode 0 #math, 1 #png; z_0 = 0, d = 0.001,
+ * scale f32(w/2) as in fut/voxel_renderer.fut:101-104. */
+int      fso_render_state(const fso_camera *cam, int32_t mode, int32_t smoothing_on,
+                          const uint32_t *shaded, int32_t hc, int32_t wc,
+                          const int32_t *alt, int32_t ha, int32_t wa, const float sun[3],
+                          int32_t h, int32_t w, uint32_t *out);
+
 /* ---- stateless benchmark program (tests/futspace.fut:47-145) ---- */
 int fso_render_nearest(const fso_camera *cam, const int32_t *col, const int32_t *alt,
                        int32_t q, int32_t r, int32_t l, int32_t m, int32_t sky,

@@ -57,6 +57,10 @@ _SIG = {
     "fso_sun": (None, [F, F, P]),
     "fso_shadowmap": (ctypes.c_int, [P, I, I, P, I, I, P, I, P]),
     "fso_shadow_counts": (ctypes.c_int, [P, I, I, P, I, P]),
+    "fso_expo2": (F, [F, F]),
+    "fso_function_coloring": (U, [F, F]),
+    "fso_math_shadowmap": (ctypes.c_int, [P, P]),
+    "fso_render_state": (ctypes.c_int, [CP, I, I, P, I, I, P, I, I, P, I, I, P]),
     "fso_sky": (U, [F]),
     "fso_render_nearest": (ctypes.c_int, [CP, P, P, I, I, I, I, I, P]),
     "fso_render_nearest_painter": (ctypes.c_int, [CP, P, P, I, I, I, I, I, P]),
@@ -179,6 +183,28 @@ def shade_factor(counts):
     f = np.float32
     return np.sqrt(f(1.0) / (f(4.0) * np.asarray(counts).astype(np.float32) + f(1.0))).astype(
         np.float32)
+
+
+def math_shadowmap(sun_vec):
+    """The #math mode's shaded map (function_coloring over expo2), 1024^2."""
+    s = _f32(sun_vec)
+    out = np.empty((1024, 1024), dtype=np.uint32)
+    assert lib().fso_math_shadowmap(s.ctypes.data, out.ctypes.data) == 0
+    return out
+
+
+def render_state(view, mode, smoothing_on, shaded, alt, sun_vec):
+    """The state's render entry (fut/interactive.fut:173-183): mode "math"
+    or "png", smoothing on/off, z_0 = 0 and d = 0.001; view supplies the
+    camera and frame size (its depth fields are not used)."""
+    c, a, s = _u32(shaded), _i32(alt), _f32(sun_vec)
+    out = np.empty((view.h, view.w), dtype=np.uint32)
+    rc = lib().fso_render_state(ctypes.byref(camera(view)), 1 if mode == "png" else 0,
+                                1 if smoothing_on else 0, c.ctypes.data, c.shape[0], c.shape[1],
+                                a.ctypes.data, a.shape[0], a.shape[1], s.ctypes.data,
+                                view.h, view.w, out.ctypes.data)
+    assert rc == 0
+    return out
 
 
 def update_map(color, height, sun_height=0.1, sun_ang=0.1, size=1024, bits=8):

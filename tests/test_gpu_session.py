@@ -107,3 +107,33 @@ def test_mismatched_update_map_reports_error(ctx):
     with pytest.raises(RuntimeError, match="differ"):
         s.update_map(np.zeros((4, 4), np.uint32), np.zeros((4, 5), np.int32))
     s.close()
+
+
+def _toggle(s, key):
+    s.key(True, key)
+    s.step()
+    s.key(False, key)
+
+
+@pytest.mark.parametrize("keys,mode,smooth", [(("2",), "png", True), (("1",), "math", False),
+                                              (("1", "2"), "math", True)])
+def test_math_mode_and_smoothing_match_the_reference_render(ctx, keys, mode, smooth):
+    """Key 1 toggles #png/#math, key 2 the smoothing (fut/interactive.fut:146-159);
+    render follows fut/interactive.fut:173-183 for every combination."""
+    col, hts = scene.scene_maps(21, 256)
+    s = Session(ctx)
+    s.resize(70, 90)
+    s.update_map(col, hts)
+    for k in keys:
+        _toggle(s, k)
+    s.key(True, "w")          # move, so the camera is not on a grid line of #math
+    s.step()
+    s.key(False, "w")
+    _, _, _, _, _, _, sun_h, sun_a, _ = s.text_content()
+    sky = O.lib().fso_sky(np.float32(sun_h))
+    shaded, alt = O.update_map(col, hts, size=1024)
+    view = _state_view(s, sky, 70, 90)
+    want = O.render_state(view, mode, smooth, shaded, alt, O.sun(sun_h, sun_a))
+    got = s.render()
+    assert np.array_equal(got, want)
+    s.close()

@@ -388,3 +388,49 @@ def test_golden_flat_map_has_no_shadow():
     colour, _, flat = _golden()
     shaded, alt = O.update_map(colour, flat, size=64)
     assert not O.shadow_counts(alt, O.sun(0.1, 0.1), 64).any()
+
+
+# ---- the state's render: #math mode and #on smoothing -------------------
+
+def test_state_render_modes_reduce_to_the_sample_under_the_camera():
+    """render fixes z_0 = 0 (fut/voxel_renderer.fut:103): the first slice is
+    at z = 0, so for both modes and both smoothing branches each frame is the
+    colour under the camera (the sky when it is 0)."""
+    col, hts = scene.scene_maps(5, 64)
+    shaded, alt = O.update_map(col, hts, size=64)
+    rng = np.random.default_rng(12)
+    for _ in range(4):
+        h, a = float(rng.uniform(-0.5, 1.2)), float(rng.uniform(-3, 3))
+        sv = O.sun(h, a)
+        v = View(x=float(rng.uniform(-900, 900)), y=float(rng.uniform(-900, 900)),
+                 height=float(rng.uniform(50, 400)), angle=float(rng.uniform(-4, 4)),
+                 horizon=float(rng.uniform(-100, 300)), distance=float(rng.uniform(50, 900)),
+                 fov=1.2, sky_color=0xFF102030, h=23, w=31)
+        mm = O.math_shadowmap(sv)
+        for mode in ("png", "math"):
+            off = O.render_state(v, mode, False, shaded, alt, sv)
+            on = O.render_state(v, mode, True, shaded, alt, sv)
+            assert np.array_equal(off, on)
+            assert np.all(off == off[0, 0])
+            if mode == "math":
+                c = mm[int(np.trunc(v.y)) % 1024, int(np.trunc(v.x)) % 1024]
+            else:
+                c = O.png_color_filtered(shaded, v.x, v.y)
+            assert off[0, 0] == (c if c else v.sky_color)
+        assert np.array_equal(O.render_state(v, "png", False, shaded, alt, sv),
+                              O.render_off(View(**{**v.__dict__, "d": 0.001, "z0": 0.0,
+                                                   "scale": 0.0, "n_slices": 0}), shaded, alt))
+
+
+def test_math_samplers():
+    f = np.float32
+    for x, y in [(3.5, -7.25), (1023.0, 2.0), (-1023.5, 100.0), (1024.0, 0.5), (0.0, -1024.0)]:
+        want = (f(f(f(x) * f(x)) - f(f(y) * f(y))) / f(1024)) if (-1024 < x < 1024 and
+                                                                  -1024 < y < 1024) else f(0)
+        assert L.fso_expo2(x, y) == pytest.approx(float(want), rel=1e-6, abs=1e-6)
+    assert L.fso_function_coloring(64.0, 3.5) == 0xFF000000
+    assert L.fso_function_coloring(-128.9, 3.5) == 0xFF000000      # truncation to -128
+    assert L.fso_function_coloring(65.0, 3.5) == 0xFFFFFFFF
+    # flat regions of expo2 never shadow: a sun straight up counts nothing
+    mm = O.math_shadowmap(O.sun(0.0, 0.0))
+    assert set(np.unique(mm).tolist()) <= {0xFF000000, 0xFFFFFFFF}
