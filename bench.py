@@ -350,11 +350,11 @@ def main():
     if not args.no_batch:
         ctx.set_stream(None)
         rec["batched"] = bench_batched(ctx, max(3, args.batch_reps if not args.quick else 3))
+    ctx.set_stream(None)
     if not args.no_diff:
-        ctx.set_stream(None)
         rec["diff"] = bench_diff(ctx, 50 if not args.quick else 5)
-        if not args.no_fit:
-            rec["fit"] = bench_fit(ctx, args.fit_steps if not args.quick else 3)
+    if not args.no_fit:
+        rec["fit"] = bench_fit(ctx, args.fit_steps if not args.quick else 3)
     if args.calib_copy:
         calib_copy()
     if not args.no_cpu:

@@ -55,7 +55,10 @@ struct futhark_context {
     int prof_on;              /* bracket render kernels with events */
     int prof_n;
     hipEvent_t *prof_ev;      /* FS_PROF_MAX pairs */
+    float *d_partials;        /* per-workgroup partial sums of the reductions */
 };
+
+#define FS_PARTIALS 16384     /* floats in d_partials */
 
 #define FS_PROF_MAX 4096
 
@@ -274,6 +277,8 @@ void futhark_context_free(struct futhark_context *ctx)
                 hipEventDestroy(ctx->prof_ev[i]);
         free(ctx->prof_ev);
     }
+    if (ctx->d_partials)
+        hipFree(ctx->d_partials);
     if (ctx->own_stream)
         hipStreamDestroy(ctx->own_stream);
     free(ctx->error);
@@ -1490,6 +1495,22 @@ int fsx_map_shade_factor(struct futhark_context *ctx, const struct fsx_map *map,
     return 0;
 }
 
+/* The context's scratch for per-workgroup partial sums (allocated once;
+ * stream-ordered reuse). */
+static int partials(struct futhark_context *ctx, float **p)
+{
+    if (!ctx->d_partials) {
+        hipError_t e = hipMalloc((void **)&ctx->d_partials, sizeof(float) * FS_PARTIALS);
+        if (e != hipSuccess) {
+            ctx->d_partials = NULL;
+            set_error(ctx, "scratch allocation failed: %s", hipGetErrorString(e));
+            return 1;
+        }
+    }
+    *p = ctx->d_partials;
+    return 0;
+}
+
 static int diff_frame(struct futhark_context *ctx, const char *what, const struct fsx_view *v,
                       int32_t hc, int32_t wc, fsk_frame *f)
 {
@@ -1558,9 +1579,12 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
         HIPCHECK(ctx, fsk_zero_f32(dev_gplanes, (size_t)3 * (size_t)hc * (size_t)wc, ctx->stream));
         HIPCHECK(ctx, fsk_zero_f32(dev_gsky, 1, ctx->stream));
     }
+    float *part;
+    if (partials(ctx, &part))
+        return 1;
     if (view->h > 0 && view->w > 0)
         HIPCHECK(ctx, fsk_diff_grad(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg,
-                                    dev_gplanes, dev_gsky, ctx->stream));
+                                    dev_gplanes, dev_gsky, part, FS_PARTIALS, ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }
@@ -1576,9 +1600,11 @@ int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *
         return 1;
     }
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    if (!accumulate)
-        HIPCHECK(ctx, fsk_zero_f32(dev_loss, 1, ctx->stream));
-    HIPCHECK(ctx, fsk_l2_loss(dev_img, dev_target, (size_t)n, dev_grad, dev_loss, ctx->stream));
+    float *part;
+    if (partials(ctx, &part))
+        return 1;
+    HIPCHECK(ctx, fsk_l2_loss(dev_img, dev_target, (size_t)n, dev_grad, dev_loss, accumulate, part,
+                              FS_PARTIALS, ctx->stream));
     return 0;
 }
 

@@ -116,11 +116,14 @@ int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int
                    int32_t wc, float sky_scale, const int32_t *d_src, float *d_img, hipStream_t s);
 int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                   int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
-                  float *d_gsky, hipStream_t s);
+                  float *d_gsky, float *d_partials, int32_t n_partials, hipStream_t s);
 int fsk_zero_f32(float *d_p, size_t n, hipStream_t s);
 int fsk_fill_u32(uint32_t *d_p, size_t n, uint32_t v, hipStream_t s);
+/* Reductions write one partial sum per workgroup into d_partials (at most
+ * n_partials) and a one-workgroup pass adds their sum, in a fixed order,
+ * to the scalar: no atomics on the scalar, reproducible results. */
 int fsk_l2_loss(const float *d_img, const float *d_tgt, size_t n, float *d_g, float *d_loss,
-                hipStream_t s);
+                int32_t accumulate, float *d_partials, int32_t n_partials, hipStream_t s);
 int fsk_shade_factor(const uint16_t *d_hplain, int32_t hrows, int32_t hcols, float sun0,
                      float sun1, float sun2, int32_t size, float *d_sf, hipStream_t s);
 

@@ -1297,6 +1297,7 @@ struct DiffArgs {
     float *img;            /* forward output [3][rows][cols] */
     float *gcp;            /* backward: dL/dcp accumulated [3][hc][wc] */
     float *gsky;           /* backward: dL/dsky_scale accumulated */
+    float *partials;       /* backward: per-workgroup sky terms */
 };
 
 __device__ __forceinline__ float mixf(float m1, float a, float m2, float b)
@@ -1421,19 +1422,19 @@ constexpr int kGradRows = 2;           /* rows per thread in the adjoint */
 constexpr int kGradBlockRows = (kGradThreads / 64) * kGradRows;   /* 32 */
 constexpr int kGradCapLog = 12;
 constexpr int kGradCap = 1 << kGradCapLog;   /* texel slots of a workgroup's LDS table */
-constexpr int kGradProbes = 8;
+constexpr int kGradProbes = 2;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
 /* Add v[0..2] to texel `key` of the workgroup's LDS table (open addressing,
  * linear probing).  A texel that finds no slot within kGradProbes, or any
- * texel once the table is three quarters full (far-range blocks, where
+ * texel once a quarter of the table is taken (far-range blocks, where
  * every sample has texels of its own), goes straight to the global
  * gradient. */
 __device__ __forceinline__ void table_add(uint32_t *keys, float *vals, uint32_t *fill, float *gcp,
                                           size_t plane, uint32_t key, const float v[3])
 {
     uint32_t h = (key * 2654435761u) >> (32 - kGradCapLog);
-    const bool open = *(volatile uint32_t *)fill < (uint32_t)(kGradCap * 3 / 4);
+    const bool open = *(volatile uint32_t *)fill < (uint32_t)(kGradCap / 4);
 #pragma unroll 1
     for (int p = 0; open && p < kGradProbes; p++) {
         uint32_t k = keys[h];
@@ -1557,12 +1558,19 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
             run[0] = run[1] = run[2] = 0.0f;
         }
     }
+    __shared__ float wsky[kGradThreads / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
         sky_acc += __shfl_xor(sky_acc, off);
-    if ((threadIdx.x & 63) == 0 && sky_acc != 0.0f)
-        unsafeAtomicAdd(a.gsky, sky_acc);
+    if ((threadIdx.x & 63) == 0)
+        wsky[threadIdx.x >> 6] = sky_acc;
     __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kGradThreads / 64; w++)
+            t += wsky[w];
+        a.partials[blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
     for (int32_t i = threadIdx.x; i < kGradCap; i += kGradThreads) {
         const uint32_t key = keys[i];
         if (key == kNoKey)
@@ -1576,13 +1584,36 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
     }
 }
 
-/* L2 loss of the fitting harness: g = 2 * (img - target) and
- * loss += sum((img - target)^2), one pass, one atomic per workgroup. */
+/* Sum of n per-workgroup partials in a fixed order (one workgroup),
+ * stored into *dst or added to it. */
+__global__ __launch_bounds__(1024) void k_sum_partials(const float *__restrict__ part, int32_t n,
+                                                       float *__restrict__ dst, int32_t accumulate)
+{
+    __shared__ float wsum[16];
+    float acc = 0.0f;
+    for (int32_t i = threadIdx.x; i < n; i += 1024)
+        acc += part[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0)
+        wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < 16; w++)
+            t += wsum[w];
+        *dst = accumulate ? *dst + t : t;
+    }
+}
+
+/* L2 loss of the fitting harness: g = 2 * (img - target) and one partial
+ * sum((img - target)^2) per workgroup. */
 __global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
                                                  const float *__restrict__ tgt, size_t n,
-                                                 float *__restrict__ g, float *__restrict__ loss)
+                                                 float *__restrict__ g, float *__restrict__ part)
 {
-    __shared__ float part[4];
+    __shared__ float wpart[4];
     float acc = 0.0f;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t n4 = n / 4;
@@ -1601,10 +1632,10 @@ __global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
     for (int off = 32; off > 0; off >>= 1)
         acc += __shfl_xor(acc, off);
     if ((threadIdx.x & 63) == 0)
-        part[threadIdx.x >> 6] = acc;
+        wpart[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0)
-        unsafeAtomicAdd(loss, (part[0] + part[1]) + (part[2] + part[3]));
+        part[blockIdx.x] = (wpart[0] + wpart[1]) + (wpart[2] + wpart[3]);
 }
 
 /* A frame of one colour (the degenerate frames of the state's #math mode). */
@@ -1740,6 +1771,7 @@ static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, 
     a.img = nullptr;
     a.gcp = nullptr;
     a.gsky = nullptr;
+    a.partials = nullptr;
     return a;
 }
 
@@ -1777,17 +1809,19 @@ int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int
 }
 
 int fsk_l2_loss(const float *d_img, const float *d_tgt, size_t n, float *d_g, float *d_loss,
-                hipStream_t s)
+                int32_t accumulate, float *d_partials, int32_t n_partials, hipStream_t s)
 {
-    if (n == 0)
-        return hipSuccess;
     size_t blocks = (n / 4 + 255) / 256;
     if (blocks > 4096)
         blocks = 4096;
     if (blocks == 0)
         blocks = 1;
+    if ((int64_t)blocks > n_partials)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_l2_loss, dim3((unsigned)blocks), dim3(256), 0, s, d_img, d_tgt, n, d_g,
-                       d_loss);
+                       d_partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials, (int32_t)blocks,
+                       d_loss, accumulate);
     return hipGetLastError();
 }
 
@@ -1815,7 +1849,7 @@ int fsk_zero_f32(float *d_p, size_t n, hipStream_t s)
 
 int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                   int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
-                  float *d_gsky, hipStream_t s)
+                  float *d_gsky, float *d_partials, int32_t n_partials, hipStream_t s)
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
@@ -1823,8 +1857,13 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
     a.gimg = d_gimg;
     a.gcp = d_gcp;
     a.gsky = d_gsky;
+    a.partials = d_partials;
     dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
+    if ((int64_t)grid.x * grid.y > n_partials)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_diff_grad, grid, dim3(kGradThreads), 0, s, a);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials,
+                       (int32_t)(grid.x * grid.y), d_gsky, 1);
     return hipGetLastError();
 }
 
