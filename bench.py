@@ -176,32 +176,42 @@ def bench_diff(ctx, reps):
         r.forward_raw(planes, scene.SUN_HEIGHT, v, img, src)
 
     def step():
+        # forward + L2 loss + adjoint (gradient cleared first), one call
+        r.l2_step(planes, scene.SUN_HEIGHT, v, target, img, src, gimg, gplanes, gsky, loss)
+        return loss
+
+    def step_separate():
         r.forward_raw(planes, scene.SUN_HEIGHT, v, img, src)
         r.l2_loss_grad(img, target, gimg, loss)
         r.vjp_raw(planes, src, gimg, v, gplanes, gsky)
         return loss
 
+    def wall(fn):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / reps
+
     for _ in range(5):
         step()
+        step_separate()
     torch.cuda.synchronize()
     fwd_kernel_ms = _kernel_ms(ctx, fwd, reps)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        fwd()
-    torch.cuda.synchronize()
-    fwd_ms = 1e3 * (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        loss = step()
-    torch.cuda.synchronize()
-    step_ms = 1e3 * (time.perf_counter() - t0) / reps
+    step_kernel_ms = _kernel_ms(ctx, step, reps)
+    fwd_ms = wall(fwd)
+    step_ms = wall(step)
+    separate_ms = wall(step_separate)
     # the adjoint alone (clear + scatter-add), kernel events
     vjp_kernel_ms = _kernel_ms(ctx, lambda: r.vjp_raw(planes, src, gimg, v, gplanes, gsky), reps)
+    step()
     visible = int((src >= 0).sum())
     return {"workload": "1280x720 view, 2048^2 map, 2048 slices, grad w.r.t. f32 "
                         "[3,2048,2048] colour planes + (sun_height, sun_ang)",
             "fwd_ms": fwd_ms, "fwd_bwd_step_ms": step_ms, "step_over_fwd": step_ms / fwd_ms,
-            "fwd_kernel_ms": fwd_kernel_ms, "vjp_kernel_ms": vjp_kernel_ms,
+            "fwd_kernel_ms": fwd_kernel_ms, "step_kernel_ms": step_kernel_ms,
+            "step_over_fwd_kernels": step_kernel_ms / fwd_kernel_ms,
+            "separate_calls_step_ms": separate_ms, "vjp_kernel_ms": vjp_kernel_ms,
             "loss": float(loss[0]), "terrain_pixels": visible}
 
 

@@ -804,6 +804,7 @@ static void frame_of_view(const struct fsx_view *v, fsk_frame *f)
     f->rows = v->h;
     f->cols = v->w;
     f->sky = v->sky_color;
+    f->tame = 0;
 }
 
 static int render_into(struct futhark_context *ctx, const struct fs_terrain *t,
@@ -1575,16 +1576,71 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
     }
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
-    if (!accumulate) {
+    if (!accumulate)
         HIPCHECK(ctx, fsk_zero_f32(dev_gplanes, (size_t)3 * (size_t)hc * (size_t)wc, ctx->stream));
-        HIPCHECK(ctx, fsk_zero_f32(dev_gsky, 1, ctx->stream));
-    }
     float *part;
     if (partials(ctx, &part))
         return 1;
     if (view->h > 0 && view->w > 0)
         HIPCHECK(ctx, fsk_diff_grad(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg,
-                                    dev_gplanes, dev_gsky, part, FS_PARTIALS, ctx->stream));
+                                    dev_gplanes, dev_gsky, accumulate != 0, part, FS_PARTIALS,
+                                    ctx->stream));
+    else if (!accumulate)
+        HIPCHECK(ctx, fsk_zero_f32(dev_gsky, 1, ctx->stream));
+    prof_end(ctx, pe);
+    return 0;
+}
+
+int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
+                           const struct fsx_view *view, const float *dev_planes,
+                           const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
+                           const float *dev_target, int32_t accumulate, float *dev_img,
+                           int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
+                           float *dev_gsky, float *dev_loss)
+{
+    if (!ctx || !map || !view)
+        return 1;
+    fsk_frame f;
+    if (diff_frame(ctx, "fsx_render_f32_l2_step", view, hc, wc, &f))
+        return 1;
+    if (!dev_planes || !dev_sf || !dev_gplanes || !dev_gsky || !dev_loss ||
+        ((!dev_target || !dev_img || !dev_src || !dev_gimg) && view->h > 0 && view->w > 0)) {
+        set_error(ctx, "fsx_render_f32_l2_step: null buffer");
+        return 1;
+    }
+    const size_t nplanes = (size_t)3 * (size_t)hc * (size_t)wc;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int pe = prof_begin(ctx);
+    if (view->h == 0 || view->w == 0) {
+        if (!accumulate) {
+            HIPCHECK(ctx, fsk_zero_f32(dev_gplanes, nplanes, ctx->stream));
+            HIPCHECK(ctx, fsk_zero_f32(dev_gsky, 1, ctx->stream));
+            HIPCHECK(ctx, fsk_zero_f32(dev_loss, 1, ctx->stream));
+        }
+        prof_end(ctx, pe);
+        return 0;
+    }
+    float *part;
+    if (partials(ctx, &part))
+        return 1;
+    fsk_map m;
+    m.hq = map->t->d_hq;
+    m.hrows = map->t->rows;
+    m.hcols = map->t->cols;
+    m.hbits = map->t->hbits;
+    m.cq = NULL;
+    m.crows = hc;
+    m.ccols = wc;
+    HIPCHECK(ctx, fsk_render_slices(&m, &f, dev_src, view->w, ctx->stream));
+    /* forward colour, loss terms and (unless accumulating) the clear of the
+     * gradient planes in one launch; the loss partials are summed before
+     * the adjoint reuses the scratch */
+    HIPCHECK(ctx, fsk_diff_color_l2(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_target,
+                                    dev_img, dev_gimg, dev_loss, accumulate != 0,
+                                    accumulate ? NULL : dev_gplanes, nplanes, part, FS_PARTIALS,
+                                    ctx->stream));
+    HIPCHECK(ctx, fsk_diff_grad(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg, dev_gplanes,
+                                dev_gsky, accumulate != 0, part, FS_PARTIALS, ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }
@@ -1751,6 +1807,7 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
         hv[i].m.crows = m->s->rows;
         hv[i].m.ccols = m->s->cols;
         frame_of_view(&views[i], &hv[i].f);
+        hv[i].f.tame = fsk_frame_tame(&hv[i].f);
         hv[i].offset = offsets[i];
         hv[i].ntiles = (views[i].w + tc - 1) / tc;
         n_items += hv[i].ntiles;

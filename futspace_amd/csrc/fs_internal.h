@@ -35,7 +35,14 @@ typedef struct {
     float d, z0, scale, fw;
     int32_t n, rows, cols;
     uint32_t sky;
+    int32_t tame;        /* fsk_frame_tame(this); set by the launchers */
 } fsk_frame;
+
+/* 1 when every sample of the frame stays where the render kernel may skip
+ * the out-of-range branches of the x86 casts: finite scalars, slice depths
+ * increasing from z0 > 0, every sample coordinate below 2^22 and every
+ * projected row below 2^30 in magnitude for any 16-bit height. */
+int32_t fsk_frame_tame(const fsk_frame *f);
 
 /* Plain render of one frame into out[rows][pitch]; the kernel builds its
  * own slice table per depth chunk. */
@@ -116,7 +123,16 @@ int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int
                    int32_t wc, float sky_scale, const int32_t *d_src, float *d_img, hipStream_t s);
 int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                   int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
-                  float *d_gsky, float *d_partials, int32_t n_partials, hipStream_t s);
+                  float *d_gsky, int32_t sky_accumulate, float *d_partials, int32_t n_partials,
+                  hipStream_t s);
+/* Forward colour + L2 loss terms of the fused step: img, g = 2 (img - tgt)
+ * and *d_loss (= or += sum (img - tgt)^2); clears d_zero[zero_n] (if not
+ * null) in the same launch. */
+int fsk_diff_color_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                      int32_t wc, float sky_scale, const int32_t *d_src, const float *d_tgt,
+                      float *d_img, float *d_g, float *d_loss, int32_t loss_accumulate,
+                      float *d_zero, size_t zero_n, float *d_partials, int32_t n_partials,
+                      hipStream_t s);
 int fsk_zero_f32(float *d_p, size_t n, hipStream_t s);
 int fsk_fill_u32(uint32_t *d_p, size_t n, uint32_t v, hipStream_t s);
 /* Reductions write one partial sum per workgroup into d_partials (at most

@@ -21,6 +21,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <cmath>
+#include <type_traits>
+
 #include "fs_internal.h"
 
 namespace {
@@ -160,14 +163,16 @@ __device__ __forceinline__ uint32_t argb_mix(float m1, uint32_t c1, float m2, ui
 
 /* png_height_filtered, fut/render_functions.fut:67-77, from the height
  * quad at (floor y, floor x). */
-template <int HBITS, bool HP2>
+template <int HBITS, bool HP2, bool TAME = false>
 __device__ __forceinline__ uint2 height_quad(const fsk_map &m, float x, float y)
 {
-    int32_t ix = wrap<HP2>(idx_of(floorf(x)), m.hcols);
-    int32_t iy = wrap<HP2>(idx_of(floorf(y)), m.hrows);
+    /* TAME: |x|, |y| < 2^22, so floor converts exactly without a clamp */
+    int32_t ix = wrap<HP2>(TAME ? (int32_t)floorf(x) : idx_of(floorf(x)), m.hcols);
+    int32_t iy = wrap<HP2>(TAME ? (int32_t)floorf(y) : idx_of(floorf(y)), m.hrows);
     /* maps are at most 2^28 texels (host-checked), so 32-bit offsets; the
      * wrapped index is always inside the map, whatever x and y are */
-    uint32_t t = (uint32_t)iy * (uint32_t)m.hcols + (uint32_t)ix;
+    uint32_t t = HP2 ? ((uint32_t)iy << __builtin_ctz((uint32_t)m.hcols)) + (uint32_t)ix
+                     : (uint32_t)iy * (uint32_t)m.hcols + (uint32_t)ix;
     if (HBITS == 8)
         return make_uint2(*(const uint32_t *)((const char *)m.hq + t * 4u), 0u);
     return *(const uint2 *)((const char *)m.hq + t * 8u);
@@ -240,20 +245,18 @@ __device__ __forceinline__ Slice slice_params(const fsk_frame &f, int32_t k)
 constexpr int kMaxWaves = 16;                    /* 1024-thread workgroups */
 constexpr int kChunk = 2048;                     /* slices per chunk, at most */
 
-/* Skewed slice-table entries for chunks of up to `chunk` slices. */
+/* Slice-table slots for chunks of up to `chunk` slices: the table is slot
+ * major, slot j * SG + g holding slice g * kr + j of the chunk (slice group
+ * g, its j-th slice), so that the slice groups of a wave read adjacent
+ * records at an offset fixed per j.  Chunks are whole multiples of SG * 4
+ * slots (the host rounds them), which covers the gather groups' reads past
+ * kr. */
 __host__ __device__ constexpr int tab_stride(int chunk)
 {
-    return chunk + chunk / 16;
+    return chunk;
 }
 constexpr int kMaxRows = 4096;                   /* queue entries keep the row in 12 bits */
-
-/* Slice s of a chunk lives at s + s/16: the 16 slice groups of a wave
- * read entries 8 apart, which the skew spreads so that a float4 read
- * needs only the two passes its 256 bytes take anyway. */
-__device__ __forceinline__ int32_t skew(int32_t s)
-{
-    return s + (s >> 4);
-}
+constexpr int kChunkAlign = 4;                   /* chunks are multiples of SG * kChunkAlign */
 
 /* blockIdx -> column tile.  Workgroups are dispatched round-robin over
  * the 8 XCDs (workgroup i runs on XCD i mod 8), so tile ranges are dealt
@@ -495,7 +498,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     constexpr int NW = NT / 64;                  /* waves */
     constexpr int SG = NT / TC;                  /* slice groups */
     constexpr int KR = CMAX / SG;                /* slices per lane per chunk, at most */
-    static_assert(NW <= kMaxWaves && KR % GATHER == 0 && GATHER % 2 == 0 && KR <= 32,
+    static_assert(NW <= kMaxWaves && KR % GATHER == 0 && GATHER % 2 == 0 && KR <= 32 &&
+                      SG <= 256 && kChunkAlign % GATHER == 0,
                   "tile shape");
     const int32_t tabs = tab_stride(chunk_max);
     float4 *tl = (float4 *)lds;                                  /* [tabs] */
@@ -531,10 +535,13 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 
     for (int32_t base = 0; base < n; base += csz) {
         const int32_t cn = min(csz, n - base);
-        /* phase 0: the chunk's slice records, computed here or copied
-         * from the view's precomputed table (gtl/gtz, same values) */
-        for (int32_t s = tid; s < cn; s += NT) {
-            int32_t o = skew(s);
+        /* phase 0: the chunk's slice records in slot order (slot j*SG + g
+         * holds slice g*kr + j; slots past the chunk repeat its last
+         * slice), computed here or copied from the view's precomputed
+         * table (gtl/gtz, same values) */
+        const int32_t nslots = ((kr + GATHER - 1) / GATHER) * GATHER * SG;
+        for (int32_t o = tid; o < nslots; o += NT) {
+            const int32_t s = min((o % SG) * kr + o / SG, cn - 1);
             if (gtl) {
                 tl[o] = gtl[base + s];
                 tz[o] = gtz[base + s];
@@ -550,51 +557,78 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 
         /* phase 1: rows are < 2^16, two per register.  Branch-free in
          * groups of GATHER samples: all of a group's height gathers are
-         * issued before the first is used (samples past the chunk or the
-         * frame read a valid table entry and an in-map texel, then count
-         * as row H) */
+         * issued before the first is used.  Samples past the chunk, the
+         * slice group or the frame read a defined table slot and an in-map
+         * texel; the fix-up below raises them to row H. */
         uint32_t ah[KR / 2];
         int32_t lmin = H;
+        const float4 *tlg = tl + g;
+        const float *tzg = tz + g;
+        auto project = [&](auto tame_tag, auto full_tag) {
+            constexpr bool TAME = decltype(tame_tag)::value;
+            constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
-        for (int j0 = 0; j0 < KR; j0 += GATHER) {
-            /* kr is uniform over the workgroup: whole groups past it are
-             * skipped, their rows stay H */
-            if (j0 >= kr) {
+            for (int j0 = 0; j0 < KR; j0 += GATHER) {
+                /* kr is uniform over the workgroup: whole groups past it
+                 * are skipped, their rows stay H.  A full chunk (kr == KR)
+                 * has no such group, and no branch between the groups, so
+                 * the gathers of later groups may be issued early */
+                if (!FULL && j0 >= kr) {
 #pragma unroll
-                for (int u = 0; u < GATHER; u += 2)
-                    ah[(j0 + u) / 2] = (uint32_t)H | ((uint32_t)H << 16);
-                continue;
+                    for (int u = 0; u < GATHER; u += 2)
+                        ah[(j0 + u) / 2] = (uint32_t)H | ((uint32_t)H << 16);
+                    continue;
+                }
+                float xs[GATHER], ys[GATHER], izs[GATHER];
+                uint2 qs[GATHER];
+#pragma unroll
+                for (int u = 0; u < GATHER; u++) {
+                    float4 L = tlg[(j0 + u) * SG];
+                    xs[u] = L.x + (fcol * L.z);
+                    ys[u] = L.y + (fcol * L.w);
+                    izs[u] = tzg[(j0 + u) * SG];
+                    if (MODE == kOutNearest)
+                        qs[u] = make_uint2(
+                            ((const uint32_t *)m.hq)[nearest_index<HP2>(m, xs[u], ys[u])], 0u);
+                    else
+                        qs[u] = height_quad<HBITS, HP2, TAME>(m, xs[u], ys[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < GATHER; u++) {
+                    const int j = j0 + u;
+                    float hgt = MODE == kOutNearest ? (float)(int32_t)qs[u].x
+                                                    : height_interp<HBITS>(qs[u], xs[u], ys[u]);
+                    float rel = ((f.cam_h - hgt) * izs[u]) + f.horizon;
+                    /* a tame frame keeps |rel| < 2^30: the cast of the
+                     * clamped value is the whole x86 semantics */
+                    int32_t r = TAME ? (int32_t)__builtin_amdgcn_fmed3f(rel, 0.0f, Hf)
+                                     : screen_row_clamped(rel, Hf);
+                    if (j & 1)
+                        ah[j / 2] |= (uint32_t)r << 16;
+                    else
+                        ah[j / 2] = (uint32_t)r;
+                    lmin = min(lmin, r);
+                }
             }
-            float xs[GATHER], ys[GATHER], izs[GATHER];
-            uint2 qs[GATHER];
+        };
+        if (MODE != kOutNearest && f.tame && kr == KR)
+            project(std::true_type{}, std::true_type{});
+        else if (MODE != kOutNearest && f.tame)
+            project(std::true_type{}, std::false_type{});
+        else
+            project(std::false_type{}, std::false_type{});
+        /* lanes holding samples past the chunk, past kr (a last gather
+         * group kr does not fill) or right of the frame: those samples
+         * become row H */
+        if (!colok || g * kr + kr > cn || (kr % GATHER) != 0) {
+            lmin = H;
 #pragma unroll
-            for (int u = 0; u < GATHER; u++) {
-                int32_t o = skew(min(g * kr + j0 + u, cn - 1));
-                float4 L = tl[o];
-                xs[u] = L.x + (fcol * L.z);
-                ys[u] = L.y + (fcol * L.w);
-                izs[u] = tz[o];
-                if (MODE == kOutNearest)
-                    qs[u] = make_uint2(
-                        ((const uint32_t *)m.hq)[nearest_index<HP2>(m, xs[u], ys[u])], 0u);
-                else
-                    qs[u] = height_quad<HBITS, HP2>(m, xs[u], ys[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < GATHER; u++) {
-                const int j = j0 + u;
-                float hgt = MODE == kOutNearest ? (float)(int32_t)qs[u].x
-                                                : height_interp<HBITS>(qs[u], xs[u], ys[u]);
-                float rel = ((f.cam_h - hgt) * izs[u]) + f.horizon;
-                /* rows are at most H: an invalid sample is raised to H
-                 * arithmetically, so that no branch splits the group */
+            for (int j = 0; j < KR; j++) {
                 const bool ok = j < kr && g * kr + j < cn && colok;
-                int32_t r = max(screen_row_clamped(rel, Hf), ok ? 0 : H);
-                if (j & 1)
-                    ah[j / 2] |= (uint32_t)r << 16;
-                else
-                    ah[j / 2] = (uint32_t)r;
-                lmin = min(lmin, r);
+                const uint32_t sh16 = (uint32_t)(j & 1) * 16u;
+                uint32_t r = ok ? (ah[j / 2] >> sh16) & 0xFFFFu : (uint32_t)H;
+                ah[j / 2] = (ah[j / 2] & ~(0xFFFFu << sh16)) | (r << sh16);
+                lmin = min(lmin, (int32_t)r);
             }
         }
 
@@ -666,8 +700,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
             for (int j = 0; j < KR; j++) {
                 if (vbits & (1u << j)) {
                     int32_t aj = (int32_t)((ah[j / 2] >> ((j & 1) * 16)) & 0xFFFFu);
-                    queue[slot] = ((uint32_t)(g * kr + j) << 16) | ((uint32_t)c << 12) |
-                                  (uint32_t)aj;
+                    queue[slot] = ((uint32_t)g << 21) | ((uint32_t)j << 16) |
+                                  ((uint32_t)c << 12) | (uint32_t)aj;
                     qbot[slot] = (uint16_t)prev;
                     prev = aj;
                     slot++;
@@ -695,8 +729,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 #pragma unroll
             for (int u = 0; u < CBATCH; u++) {
                 uint32_t e = queue[min(i0 + (uint32_t)(u * NT + tid), nq - 1u)];
-                int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
-                float4 L = tl[skew(s)];
+                int32_t cc = (int32_t)((e >> 12) & 15u);
+                float4 L = tl[(int32_t)((e >> 16) & 31u) * SG + (int32_t)(e >> 21)];
                 float fc = (float)(col0 + cc);
                 es[u] = e;
                 xs[u] = L.x + (fc * L.z);
@@ -710,7 +744,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 if (i >= nq)
                     break;
                 const uint32_t e = es[u];
-                int32_t s = (int32_t)(e >> 16), cc = (int32_t)((e >> 12) & 15u);
+                int32_t s = (int32_t)(e >> 21) * kr + (int32_t)((e >> 16) & 31u);
+                int32_t cc = (int32_t)((e >> 12) & 15u);
                 int32_t row = (int32_t)(e & 0xFFFu);
                 const float x = xs[u], y = ys[u];
                 uint32_t colour;
@@ -1112,16 +1147,16 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
             s.chunk = chunk ? chunk : kChunk;
         }
     }
-    /* a chunk holds whole slice groups; tall frames take shorter chunks
-     * until the tile fits in LDS */
-    int sg = s.nt / s.tc;
-    s.chunk = s.chunk < sg ? sg : s.chunk - s.chunk % sg;
+    /* a chunk holds whole groups of kChunkAlign slices per slice group;
+     * tall frames take shorter chunks until the tile fits in LDS */
+    int ga = (s.nt / s.tc) * kChunkAlign;
+    s.chunk = s.chunk < ga ? ga : s.chunk - s.chunk % ga;
     auto need = [&](int c) {
         return (s.tc == 8 ? render_lds_bytes_tc<8>(rows, c) : render_lds_bytes_tc<4>(rows, c)) +
                kStaticLds;
     };
-    while (s.chunk > sg && (size_t)s.wg * need(s.chunk) > lds_per_cu())
-        s.chunk = (s.chunk / 2) - (s.chunk / 2) % sg;
+    while (s.chunk > ga && (size_t)s.wg * need(s.chunk) > lds_per_cu())
+        s.chunk = (s.chunk / 2) - (s.chunk / 2) % ga;
     return s;
 }
 
@@ -1311,10 +1346,14 @@ __device__ __forceinline__ float mixf(float m1, float a, float m2, float b)
  * the bilinear weights, as png_color_filtered forms them
  * (fut/render_functions.fut:95-105) with floor-mod indices. */
 struct Bilin {
-    size_t t[4];
+    uint32_t t[4];
     float wx0, wx1, wy0, wy1;
 };
 
+/* Texel offsets are 32-bit: a plane holds at most 2^31 texels (checked by
+ * the host).  P2: both plane sides are powers of two, so the floor-mod is
+ * a mask (also for the INT_MIN of an out-of-range cast). */
+template <bool P2>
 __device__ __forceinline__ Bilin bilin_of(const fsk_frame &f, int32_t k, int32_t col, int32_t hc,
                                           int32_t wc)
 {
@@ -1323,13 +1362,13 @@ __device__ __forceinline__ Bilin bilin_of(const fsk_frame &f, int32_t k, int32_t
     float x = sl.sx + (fcol * sl.dx);
     float y = sl.sy + (fcol * sl.dy);
     float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
-    int32_t ify = fmod_i32(i32_f32_x86(fy), hc), icy = fmod_i32(i32_f32_x86(cy), hc);
-    int32_t ifx = fmod_i32(i32_f32_x86(fx), wc), icx = fmod_i32(i32_f32_x86(cx), wc);
+    int32_t ify = wrap<P2>(i32_f32_x86(fy), hc), icy = wrap<P2>(i32_f32_x86(cy), hc);
+    int32_t ifx = wrap<P2>(i32_f32_x86(fx), wc), icx = wrap<P2>(i32_f32_x86(cx), wc);
     Bilin b;
-    b.t[0] = (size_t)ify * wc + ifx;
-    b.t[1] = (size_t)ify * wc + icx;
-    b.t[2] = (size_t)icy * wc + ifx;
-    b.t[3] = (size_t)icy * wc + icx;
+    b.t[0] = (uint32_t)ify * (uint32_t)wc + (uint32_t)ifx;
+    b.t[1] = (uint32_t)ify * (uint32_t)wc + (uint32_t)icx;
+    b.t[2] = (uint32_t)icy * (uint32_t)wc + (uint32_t)ifx;
+    b.t[3] = (uint32_t)icy * (uint32_t)wc + (uint32_t)icx;
     b.wx0 = cx - x;
     b.wx1 = x - fx;
     b.wy0 = cy - y;
@@ -1364,6 +1403,7 @@ __device__ __forceinline__ void sample_colour(const DiffArgs &a, const Bilin &b,
 
 /* Forward colour, one thread per pixel (2-D blocks of 64 columns x 4
  * rows: the gathers of neighbouring pixels share texels). */
+template <bool P2>
 __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
 {
     const int32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -1381,11 +1421,74 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
             px[ch] = sky_base(ch) * a.sky_scale;
     } else {
         float sv[3][4];
-        sample_colour(a, bilin_of(a.f, k, col, a.hc, a.wc), plane, sv, px);
+        sample_colour(a, bilin_of<P2>(a.f, k, col, a.hc, a.wc), plane, sv, px);
     }
 #pragma unroll
     for (int ch = 0; ch < 3; ch++)
         a.img[ch * fplane + p] = px[ch];
+}
+
+/* First kernel of the fused L2 step: the forward colour of a 64 x 16
+ * pixel block (as k_diff_color), the loss terms g = 2 (img - target) and
+ * one partial sum of (img - target)^2 per block, summed over the block in
+ * a fixed order.  nzero workgroups, spread evenly through the grid (every
+ * `every`-th, from the first), clear the gradient planes with float4
+ * stores meanwhile, which overlaps the clear with the gather-bound colour
+ * work. */
+constexpr int kStepRows = 16;
+template <bool P2>
+__global__ __launch_bounds__(1024) void k_diff_color_l2(DiffArgs a, const float *__restrict__ tgt,
+                                                        float *__restrict__ g, int32_t nbx,
+                                                        int32_t nzero, int32_t every,
+                                                        float4 *__restrict__ zero, size_t zero_n4)
+{
+    int32_t b = blockIdx.x;
+    const int32_t zi = b / every;
+    if (zi < nzero && b % every == 0) {
+        const size_t stride = (size_t)nzero * blockDim.x;
+        for (size_t i = (size_t)zi * blockDim.x + threadIdx.x; i < zero_n4; i += stride)
+            zero[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    b -= min(zi + 1, nzero);    /* the pixel block: zero blocks before it skipped */
+    const int32_t col = (b % nbx) * 64 + (threadIdx.x & 63);
+    const int32_t row = (b / nbx) * kStepRows + (threadIdx.x >> 6);
+    const int32_t H = a.f.rows, W = a.f.cols;
+    float acc = 0.0f;
+    if (col < W && row < H) {
+        const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
+        const size_t p = (size_t)row * W + col;
+        const int32_t k = a.src[p];
+        float px[3];
+        if (k < 0) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++)
+                px[ch] = sky_base(ch) * a.sky_scale;
+        } else {
+            float sv[3][4];
+            sample_colour(a, bilin_of<P2>(a.f, k, col, a.hc, a.wc), plane, sv, px);
+        }
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float d = px[ch] - tgt[ch * fplane + p];
+            a.img[ch * fplane + p] = px[ch];
+            g[ch * fplane + p] = 2.0f * d;
+            acc += d * d;
+        }
+    }
+    __shared__ float wpart[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0)
+        wpart[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < 16; w++)
+            t += wpart[w];
+        a.partials[b] = t;
+    }
 }
 
 /* Segmented suffix sum over the lanes of a wave: lanes holding the same
@@ -1426,29 +1529,29 @@ constexpr int kGradProbes = 2;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
 /* Add v[0..2] to texel `key` of the workgroup's LDS table (open addressing,
- * linear probing).  A texel that finds no slot within kGradProbes, or any
- * texel once a quarter of the table is taken (far-range blocks, where
- * every sample has texels of its own), goes straight to the global
- * gradient. */
-__device__ __forceinline__ void table_add(uint32_t *keys, float *vals, uint32_t *fill, float *gcp,
-                                          size_t plane, uint32_t key, const float v[3])
+ * linear probing); returns whether this call took a new slot.  A texel that
+ * finds no slot within kGradProbes, or any texel once a quarter of the
+ * table is taken (far-range blocks, where every sample has texels of its
+ * own), goes straight to the global gradient. */
+__device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uint32_t *fill,
+                                          float *gcp, size_t plane, uint32_t key, const float v[3])
 {
     uint32_t h = (key * 2654435761u) >> (32 - kGradCapLog);
-    const bool open = *(volatile uint32_t *)fill < (uint32_t)(kGradCap / 4);
+    const bool open = *(const volatile uint32_t *)fill < (uint32_t)(kGradCap / 4);
 #pragma unroll 1
     for (int p = 0; open && p < kGradProbes; p++) {
         uint32_t k = keys[h];
+        bool mine = false;
         if (k == kNoKey) {
             k = atomicCAS(&keys[h], kNoKey, key);
-            if (k == kNoKey)
-                atomicAdd(fill, 1u);
+            mine = k == kNoKey;
         }
-        if (k == kNoKey || k == key) {
+        if (mine || k == key) {
 #pragma unroll
             for (int ch = 0; ch < 3; ch++)
                 if (v[ch] != 0.0f)
                     atomicAdd(&vals[ch * kGradCap + h], v[ch]);
-            return;
+            return mine;
         }
         h = (h + 1) & (kGradCap - 1);
     }
@@ -1456,6 +1559,7 @@ __device__ __forceinline__ void table_add(uint32_t *keys, float *vals, uint32_t 
     for (int ch = 0; ch < 3; ch++)
         if (v[ch] != 0.0f)
             unsafeAtomicAdd(&gcp[ch * plane + key], v[ch]);
+    return false;
 }
 
 /* Adjoint.  A workgroup covers 64 adjacent columns x 32 rows; each thread
@@ -1474,6 +1578,7 @@ __device__ __forceinline__ void table_add(uint32_t *keys, float *vals, uint32_t 
  * per-thread row runs keep many waves in flight for the gathers.  Each
  * table entry is then added into the global f32 gradient with one atomic
  * per channel.  The sky term is reduced per wave. */
+template <bool P2>
 __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
 {
     __shared__ uint32_t keys[kGradCap];
@@ -1527,7 +1632,7 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
             for (int t = 0; t < 4; t++)
                 val[t][0] = val[t][1] = val[t][2] = 0.0f;
             if (flush) {
-                Bilin b = bilin_of(a.f, k, col, a.hc, a.wc);
+                Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
                 float sv[3][4], px[3];
                 sample_colour(a, b, plane, sv, px);
                 const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
@@ -1550,8 +1655,14 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const bool head = segment_sum3(key[t], val[t]);
+                bool inserted = false;
                 if (head && key[t] != kNoKey)
-                    table_add(keys, vals, &fill, a.gcp, plane, key[t], val[t]);
+                    inserted = table_add(keys, vals, &fill, a.gcp, plane, key[t], val[t]);
+                /* the slot count is kept per wave, not per insertion: one
+                 * LDS atomic instead of one per inserting lane */
+                const uint64_t ins = __ballot(inserted);
+                if (ins && (threadIdx.x & 63) == 0)
+                    atomicAdd(&fill, (uint32_t)__popcll(ins));
             }
         }
         if (ends) {
@@ -1591,8 +1702,19 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const float *__restrict__
 {
     __shared__ float wsum[16];
     float acc = 0.0f;
-    for (int32_t i = threadIdx.x; i < n; i += 1024)
-        acc += part[i];
+    /* thread t adds part[t], part[t + 1024], ... in that order; eight
+     * loads are issued before the first add */
+    for (int32_t base = 0; base < n; base += 8 * 1024) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int32_t i = base + u * 1024 + (int32_t)threadIdx.x;
+            v[u] = i < n ? part[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            acc += v[u];
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
         acc += __shfl_xor(acc, off);
@@ -1742,6 +1864,7 @@ int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int6
     RenderArgs a;
     a.m = *m;
     a.f = *f;
+    a.f.tame = fsk_frame_tame(f);
     if (a.f.n < 0)
         a.f.n = 0;
     a.out = (uint32_t *)d_src;
@@ -1761,6 +1884,7 @@ static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, 
 {
     DiffArgs a;
     a.f = *f;
+    a.f.tame = fsk_frame_tame(f);
     a.cp = cp;
     a.sf = sf;
     a.hc = hc;
@@ -1785,6 +1909,7 @@ int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, in
     RenderArgs a;
     a.m = *m;
     a.f = *f;
+    a.f.tame = fsk_frame_tame(f);
     if (a.f.n < 0)
         a.f.n = 0;
     a.out = d_out;
@@ -1804,7 +1929,10 @@ int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int
     a.sky_scale = sky_scale;
     a.img = d_img;
     dim3 grid((f->cols + 63) / 64, (f->rows + 3) / 4);
-    hipLaunchKernelGGL(k_diff_color, grid, dim3(256), 0, s, a);
+    if (pow2(hc) && pow2(wc))
+        hipLaunchKernelGGL(k_diff_color<true>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_diff_color<false>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1849,7 +1977,8 @@ int fsk_zero_f32(float *d_p, size_t n, hipStream_t s)
 
 int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                   int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
-                  float *d_gsky, float *d_partials, int32_t n_partials, hipStream_t s)
+                  float *d_gsky, int32_t sky_accumulate, float *d_partials, int32_t n_partials,
+                  hipStream_t s)
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
@@ -1861,9 +1990,59 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
     dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
     if ((int64_t)grid.x * grid.y > n_partials)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_diff_grad, grid, dim3(kGradThreads), 0, s, a);
+    if (pow2(hc) && pow2(wc))
+        hipLaunchKernelGGL(k_diff_grad<true>, grid, dim3(kGradThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_diff_grad<false>, grid, dim3(kGradThreads), 0, s, a);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials,
-                       (int32_t)(grid.x * grid.y), d_gsky, 1);
+                       (int32_t)(grid.x * grid.y), d_gsky, sky_accumulate);
+    return hipGetLastError();
+}
+
+int fsk_diff_color_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                      int32_t wc, float sky_scale, const int32_t *d_src, const float *d_tgt,
+                      float *d_img, float *d_g, float *d_loss, int32_t loss_accumulate,
+                      float *d_zero, size_t zero_n, float *d_partials, int32_t n_partials,
+                      hipStream_t s)
+{
+    if (f->rows <= 0 || f->cols <= 0)
+        return hipSuccess;
+    DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
+    a.sky_scale = sky_scale;
+    a.img = d_img;
+    a.partials = d_partials;
+    const int32_t nbx = (f->cols + 63) / 64, nby = (f->rows + kStepRows - 1) / kStepRows;
+    const int64_t npix = (int64_t)nbx * nby;
+    if (npix > n_partials)
+        return hipErrorInvalidValue;
+    /* the clear rides along in float4 stores when it can; else it is a
+     * launch of its own */
+    const bool fused = d_zero && (((uintptr_t)d_zero) & 15u) == 0 && zero_n % 4 == 0;
+    size_t nzero = 0;
+    if (d_zero && !fused) {
+        int e = fsk_zero_f32(d_zero, zero_n, s);
+        if (e != hipSuccess)
+            return e;
+    } else if (fused) {
+        nzero = (zero_n / 4 + 4095) / 4096;
+        if (nzero > 2048)
+            nzero = 2048;
+        if (nzero == 0)
+            nzero = 1;
+    }
+    const int64_t total = npix + (int64_t)nzero;
+    const int32_t every = nzero ? (int32_t)(total / (int64_t)nzero) : (int32_t)total + 1;
+    const dim3 grid((unsigned)total);
+    float4 *z = fused ? (float4 *)d_zero : nullptr;
+    const size_t zn4 = fused ? zero_n / 4 : 0;
+    if (pow2(hc) && pow2(wc))
+        hipLaunchKernelGGL(k_diff_color_l2<true>, grid, dim3(1024), 0, s, a, d_tgt, d_g, nbx,
+                           (int32_t)nzero, every, z, zn4);
+    else
+        hipLaunchKernelGGL(k_diff_color_l2<false>, grid, dim3(1024), 0, s, a, d_tgt, d_g, nbx,
+                           (int32_t)nzero, every, z, zn4);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials, (int32_t)npix,
+                       d_loss, loss_accumulate);
     return hipGetLastError();
 }
 
@@ -1874,6 +2053,34 @@ int fsk_shade_factor(const uint16_t *d_hplain, int32_t hrows, int32_t hcols, flo
     hipLaunchKernelGGL(k_shade_factor, grid, dim3(256), 0, s, d_hplain, hrows, hcols, sun0, sun1,
                        sun2, size, d_sf);
     return hipGetLastError();
+}
+
+int32_t fsk_frame_tame(const fsk_frame *f)
+{
+    const float v[] = {f->cam_x, f->cam_y, f->cam_h, f->horizon, f->Lx, f->Ly,
+                       f->Rx,    f->Ry,    f->d,     f->z0,      f->scale, f->fw};
+    for (float x : v)
+        if (!std::isfinite(x))
+            return 0;
+    if (!(f->z0 > 0.0f) || !(f->d >= 0.0f) || !(f->scale > 0.0f) || !(f->fw >= 1.0f) ||
+        f->n <= 0)
+        return 0;
+    /* z_k = (k/2)(2 z0 + (k-1) d) is non-decreasing from z_1 = z0 (every
+     * f32 step of get_zs' term is monotone for z0 > 0, d >= 0), so
+     * inv_z <= scale / z0; a 1e-3 margin covers the roundings */
+    const double n = (double)f->n, margin = 1.001;
+    const double zmax = (n / 2.0) * (2.0 * (double)f->z0 + (n - 1.0) * (double)f->d) * margin;
+    const double izmax = (double)f->scale / (double)f->z0 * margin;
+    /* x = (lx + cam_x) + col * (rx - lx) / w over columns up to w + 7 (the
+     * last tile's lanes past the frame) */
+    const double span = ((double)f->fw + 8.0) / (double)f->fw;
+    const double ax = std::fabs((double)f->Lx), bx = std::fabs((double)f->Rx);
+    const double ay = std::fabs((double)f->Ly), by = std::fabs((double)f->Ry);
+    const double xmax = (std::fabs((double)f->cam_x) + (ax + (ax + bx) * span) * zmax) * margin;
+    const double ymax = (std::fabs((double)f->cam_y) + (ay + (ay + by) * span) * zmax) * margin;
+    const double relmax =
+        ((std::fabs((double)f->cam_h) + 65536.0) * izmax + std::fabs((double)f->horizon)) * margin;
+    return xmax < 4194304.0 && ymax < 4194304.0 && relmax < 1073741824.0;
 }
 
 int32_t fsk_render_max_rows(void)
@@ -1899,6 +2106,7 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pi
     RenderArgs a;
     a.m = *m;
     a.f = *f;
+    a.f.tame = fsk_frame_tame(f);
     if (a.f.n < 0)
         a.f.n = 0;
     a.out = d_out;

@@ -113,6 +113,38 @@ class DiffRenderer:
                 _ptr(grad), _ptr(loss)), "fsx_l2_loss")
         return grad, loss
 
+    def l2_step(self, planes, sky_scale, view: View, target, img=None, src=None, gimg=None,
+                gplanes=None, gsky=None, loss=None, accumulate=False):
+        """One fitting step in one call: forward_raw, l2_loss_grad against
+        target and vjp_raw with that gradient (fsx_render_f32_l2_step).
+        Returns (img, src, gplanes, gsky, loss); with accumulate the loss and
+        the gradients are added into the given tensors."""
+        self._check_planes(planes)
+        if tuple(target.shape) != (3, view.h, view.w) or not target.is_contiguous():
+            raise ValueError("target must be a contiguous [3][h][w] tensor")
+        if img is None:
+            img = torch.empty((3, view.h, view.w), dtype=torch.float32, device="cuda")
+        if src is None:
+            src = torch.empty((view.h, view.w), dtype=torch.int32, device="cuda")
+        if gimg is None:
+            gimg = torch.empty_like(img)
+        if gplanes is None or gsky is None or loss is None:
+            accumulate = False
+        if gplanes is None:
+            gplanes = torch.empty_like(planes)
+        if gsky is None:
+            gsky = torch.empty(1, dtype=torch.float32, device="cuda")
+        if loss is None:
+            loss = torch.empty(1, dtype=torch.float32, device="cuda")
+        cv = view.to_c()
+        with self.stream():
+            self.ctx.check(self.ctx.lib.fsx_render_f32_l2_step(
+                self.ctx.ptr, self.map.ptr, ctypes.byref(cv), _ptr(planes), _ptr(self.sf),
+                self.hc, self.wc, ctypes.c_float(sky_scale), _ptr(target),
+                1 if accumulate else 0, _ptr(img), _ptr(src), _ptr(gimg), _ptr(gplanes),
+                _ptr(gsky), _ptr(loss)), "fsx_render_f32_l2_step")
+        return img, src, gplanes, gsky, loss
+
     def render(self, planes, shading, view: View):
         """Differentiable render: img[3][h][w] as a function of planes and
         shading = (sun_height, sun_ang)."""
@@ -166,9 +198,8 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
                 img = torch.empty((3, v.h, v.w), dtype=torch.float32, device="cuda")
                 gimg = torch.empty_like(img)
                 src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
-            renderer.forward_raw(planes, sky_scale, v, img, src)
-            renderer.l2_loss_grad(img, t, gimg, losses[it:it + 1], accumulate=True)
-            renderer.vjp_raw(planes, src, gimg, v, grad, gsky, accumulate=i > 0)
+            renderer.l2_step(planes, sky_scale, v, t, img, src, gimg, grad, gsky,
+                             losses[it:it + 1], accumulate=i > 0)
         planes.grad = grad
         opt.step()
         if on_step is not None:

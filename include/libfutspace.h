@@ -231,6 +231,22 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
 int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *dev_target,
                 int64_t n, int32_t accumulate, float *dev_grad, float *dev_loss);
 
+/* One fitting step of the differentiable renderer in one call: the same
+ * results as fsx_render_f32, then fsx_l2_loss against dev_target[3][h][w]
+ * (dev_gimg = 2 (img - target), *dev_loss), then fsx_render_f32_vjp with
+ * that gradient.  accumulate = 0 overwrites *dev_loss, dev_gplanes and
+ * *dev_gsky, else adds to them.  The gradient clear shares a launch with
+ * the forward colour pass and the loss is summed per workgroup in a fixed
+ * order, so the loss of a step is reproducible; the gradients carry the
+ * f32-atomic summation order of fsx_render_f32_vjp.  Asynchronous on the
+ * context stream. */
+int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
+                           const struct fsx_view *view, const float *dev_planes,
+                           const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
+                           const float *dev_target, int32_t accumulate, float *dev_img,
+                           int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
+                           float *dev_gsky, float *dev_loss);
+
 /* Render kernel shape, as chosen by the autotuner: columns per workgroup
  * tile (4 or 8), workgroups per CU for 4-column tiles (1 or 2) and depth
  * slices per chunk (256-2048); 0 leaves a knob automatic.  Applies to

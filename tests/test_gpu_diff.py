@@ -207,3 +207,62 @@ def test_fused_l2_loss(ctx):
         assert torch.equal(g, 2.0 * (a - b))
         assert abs(float(loss[0]) - float(want)) <= 1e-5 * float(want)
         assert abs(float(loss2[0]) - 2 * float(want)) <= 2e-5 * float(want)
+
+
+@pytest.mark.parametrize("vi", range(len(VIEWS)))
+def test_fused_step_matches_separate_calls(ctx, vi):
+    """fsx_render_f32_l2_step = fsx_render_f32, fsx_l2_loss and
+    fsx_render_f32_vjp: img, src and the loss gradient bit-exact, the loss
+    within f32 summation order, the gradients within 1e-5 (f32 atomics);
+    and against the oracle's adjoint of the same loss gradient."""
+    r, col, alt, sf = _setup(ctx)
+    v = VIEWS[vi]
+    planes = planes_of(col)
+    tp = torch.from_numpy(planes).cuda()
+    rng = np.random.default_rng(200 + vi)
+    target = torch.from_numpy(rng.random((3, v.h, v.w), dtype=np.float32)).cuda()
+    img, src = r.forward_raw(tp, 0.37, v)
+    gimg, loss = r.l2_loss_grad(img, target)
+    gp, gs = r.vjp_raw(tp, src, gimg, v)
+    # stale values in the outputs: the step must overwrite them
+    f_img = torch.full_like(img, 7.0)
+    f_src = torch.full_like(src, 5)
+    f_gimg = torch.full_like(img, 3.0)
+    f_gp = torch.full_like(tp, 9.0)
+    f_gs = torch.full((1,), 4.0, device="cuda")
+    f_loss = torch.full((1,), 2.0, device="cuda")
+    r.l2_step(tp, 0.37, v, target, f_img, f_src, f_gimg, f_gp, f_gs, f_loss)
+    torch.cuda.synchronize()
+    assert torch.equal(f_img, img) and torch.equal(f_src, src) and torch.equal(f_gimg, gimg)
+    assert abs(float(f_loss[0]) - float(loss[0])) <= 1e-5 * float(loss[0])
+    assert _rel_err(f_gp.cpu().numpy(), gp.cpu().numpy()) <= 1e-5
+    _, want_src = O.render_f32(v, planes, sf, alt, 0.37)
+    g_np = gimg.cpu().numpy()
+    want_gp, want_gs = O.render_f32_vjp(v, planes, sf, want_src, g_np)
+    assert _rel_err(f_gp.cpu().numpy(), want_gp) <= 1e-5
+    sky_terms = np.abs(g_np[:, want_src < 0]).sum(axis=1) @ np.array([144, 144, 224]) / 255.0
+    assert abs(float(f_gs.cpu()[0]) - want_gs) <= 1e-5 * max(sky_terms, 1e-3)
+
+
+def test_fused_step_accumulates(ctx):
+    """accumulate adds the loss and both gradients to the given values."""
+    r, col, _, _ = _setup(ctx)
+    tp = torch.from_numpy(planes_of(col)).cuda()
+    rng = np.random.default_rng(7)
+    g = torch.empty_like(tp)
+    gs = torch.empty(1, device="cuda")
+    loss = torch.empty(1, device="cuda")
+    tot_g = torch.zeros_like(tp)
+    tot_gs = torch.zeros(1, device="cuda")
+    tot_loss = 0.0
+    for i, v in enumerate(VIEWS[:3]):
+        target = torch.from_numpy(rng.random((3, v.h, v.w), dtype=np.float32)).cuda()
+        _, _, g1, gs1, l1 = r.l2_step(tp, 0.2, v, target)
+        tot_g += g1
+        tot_gs += gs1
+        tot_loss += float(l1[0])
+        r.l2_step(tp, 0.2, v, target, None, None, None, g, gs, loss, accumulate=i > 0)
+    torch.cuda.synchronize()
+    assert torch.allclose(g, tot_g, rtol=1e-5, atol=1e-6)
+    assert abs(float(gs[0]) - float(tot_gs[0])) <= 1e-5 * max(abs(float(tot_gs[0])), 1.0)
+    assert abs(float(loss[0]) - tot_loss) <= 1e-5 * tot_loss
