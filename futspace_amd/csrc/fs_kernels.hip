@@ -200,20 +200,6 @@ __device__ __forceinline__ float height_interp(uint2 q, float x, float y)
     return ((cy - y) * x_interpolated1) + ((y - fy) * x_interpolated2);
 }
 
-/* png_color_filtered, fut/render_functions.fut:95-105. */
-template <bool CP2>
-__device__ __forceinline__ uint32_t color_bilinear(const fsk_map &m, float x, float y)
-{
-    float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
-    int32_t ix = wrap<CP2>(idx_of(fx), m.ccols);
-    int32_t iy = wrap<CP2>(idx_of(fy), m.crows);
-    uint4 q = ((const uint4 *)m.cq)[(size_t)iy * (size_t)m.ccols + (size_t)ix];
-    float wx0 = cx - x, wx1 = x - fx;
-    uint32_t x_interpolated1 = argb_mix(wx0, q.x, wx1, q.y);
-    uint32_t x_interpolated2 = argb_mix(wx0, q.z, wx1, q.w);
-    return argb_mix(cy - y, x_interpolated1, y - fy, x_interpolated2);
-}
-
 /* ------------------------------------------------------------------ */
 /* Slice parameters: get_zs term (fut/voxel_renderer.fut:34), get_h_line */
 /* (:43-60) and inv_z (:217) of slice k (0-based)                      */
@@ -645,27 +631,16 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
             excl = H;
         if (lane >= 64 - TC)
             sh.wtot[wave][c] = v;
+        /* the chunk's carry-in is read before the barrier: the last slice
+         * group overwrites it in phase 3 without a barrier in between */
+        const int32_t carry_in = sh.carry[c];
         __syncthreads();
-        if (wave == 0) {
-            /* lane = w*4 + j handles column j (and j+4 when TC == 8) of
-             * wave w: exclusive prefix over the NW waves */
-#pragma unroll
-            for (int half = 0; half < TC / 4; half++) {
-                int32_t cc = (lane & 3) + 4 * half;
-                int32_t t = (lane >> 2) < NW ? sh.wtot[lane >> 2][cc] : H;
-#pragma unroll
-                for (int off = 4; off < 64; off <<= 1) {
-                    int32_t o = __shfl_up(t, off);
-                    if (lane >= off)
-                        t = min(t, o);
-                }
-                int32_t ex = __shfl_up(t, 4);
-                if ((lane >> 2) < NW)
-                    sh.wpre[lane >> 2][cc] = min(sh.carry[cc], lane < 4 ? H : ex);
-            }
-        }
-        __syncthreads();
-        int32_t run = min(sh.wpre[wave][c], excl);
+        /* exclusive prefix over the waves: each lane reads the totals of
+         * the waves before its own (same address across the wave's lanes
+         * of one column: broadcast reads) */
+        int32_t run = min(carry_in, excl);
+        for (int w = 0; w < wave; w++)
+            run = min(run, sh.wtot[w][c]);
 
         /* phase 3: mark this lane's visible samples, reserve queue slots
          * with one wave-level prefix sum, then queue each sample with the
@@ -1332,7 +1307,7 @@ struct DiffArgs {
     float *img;            /* forward output [3][rows][cols] */
     float *gcp;            /* backward: dL/dcp accumulated [3][hc][wc] */
     float *gsky;           /* backward: dL/dsky_scale accumulated */
-    float *partials;       /* backward: per-workgroup sky terms */
+    float *partials;       /* per-workgroup partial sums (loss or sky) */
 };
 
 __device__ __forceinline__ float mixf(float m1, float a, float m2, float b)
@@ -1520,9 +1495,9 @@ __device__ __forceinline__ bool segment_sum3(uint32_t key, float v[3])
     return lane == 0 || prev != key;
 }
 
-constexpr int kGradThreads = 1024;
+constexpr int kGradThreads = 512;
 constexpr int kGradRows = 2;           /* rows per thread in the adjoint */
-constexpr int kGradBlockRows = (kGradThreads / 64) * kGradRows;   /* 32 */
+constexpr int kGradBlockRows = (kGradThreads / 64) * kGradRows;   /* 16 */
 constexpr int kGradCapLog = 12;
 constexpr int kGradCap = 1 << kGradCapLog;   /* texel slots of a workgroup's LDS table */
 constexpr int kGradProbes = 2;
@@ -1533,13 +1508,15 @@ constexpr uint32_t kNoKey = 0xFFFFFFFFu;
  * finds no slot within kGradProbes, or any texel once a quarter of the
  * table is taken (far-range blocks, where every sample has texels of its
  * own), goes straight to the global gradient. */
+template <int CAPLOG = kGradCapLog, int FILL_LIMIT = kGradCap / 4, int PROBES = kGradProbes>
 __device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uint32_t *fill,
                                           float *gcp, size_t plane, uint32_t key, const float v[3])
 {
-    uint32_t h = (key * 2654435761u) >> (32 - kGradCapLog);
-    const bool open = *(const volatile uint32_t *)fill < (uint32_t)(kGradCap / 4);
+    constexpr int kCap = 1 << CAPLOG;
+    uint32_t h = (key * 2654435761u) >> (32 - CAPLOG);
+    const bool open = *(const volatile uint32_t *)fill < (uint32_t)FILL_LIMIT;
 #pragma unroll 1
-    for (int p = 0; open && p < kGradProbes; p++) {
+    for (int p = 0; open && p < PROBES; p++) {
         uint32_t k = keys[h];
         bool mine = false;
         if (k == kNoKey) {
@@ -1550,10 +1527,10 @@ __device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uin
 #pragma unroll
             for (int ch = 0; ch < 3; ch++)
                 if (v[ch] != 0.0f)
-                    atomicAdd(&vals[ch * kGradCap + h], v[ch]);
+                    atomicAdd(&vals[ch * kCap + h], v[ch]);
             return mine;
         }
-        h = (h + 1) & (kGradCap - 1);
+        h = (h + 1) & (kCap - 1);
     }
 #pragma unroll
     for (int ch = 0; ch < 3; ch++)
@@ -1562,7 +1539,38 @@ __device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uin
     return false;
 }
 
-/* Adjoint.  A workgroup covers 64 adjacent columns x 32 rows; each thread
+/* The texel terms of one visible sample k at column col whose pixels'
+ * incoming gradients sum to run[3]: key[t] = texel t of the bilinear quad,
+ * val[t][ch] = dL/d plane_ch at that texel.  For px = mix(wy0, X1, wy1, X2),
+ * X1 = mix(wx0, s0, wx1, s1): d px / d s0 = wy0' * wx0' * s0 / px (X1
+ * cancels), likewise for the other texels, 0 where px == 0. */
+template <bool P2>
+__device__ __forceinline__ void sample_terms(const DiffArgs &a, int32_t k, int32_t col,
+                                             size_t plane, const float run[3], uint32_t key[4],
+                                             float val[4][3])
+{
+    Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
+    float sv[3][4], px[3];
+    sample_colour(a, b, plane, sv, px);
+    const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
+    const float wx0n = b.wx0 / wxs, wx1n = b.wx1 / wxs;
+    const float wy0n = b.wy0 / wys, wy1n = b.wy1 / wys;
+    const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
+    float q[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++)
+        q[ch] = px[ch] > 0.0f ? run[ch] / px[ch] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const float sf = a.sf[b.t[t]];
+        key[t] = b.t[t];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++)
+            val[t][ch] = ((q[ch] * wt[t]) * sv[ch][t]) * sf;
+    }
+}
+
+/* Adjoint.  A workgroup covers 64 adjacent columns x 16 rows; each thread
  * holds kGradRows rows of one column.  A run of equal src down the column
  * is one visible sample: its incoming gradient is summed in registers and
  * pushed through the three mixes once.  For px = mix(wy0, X1, wy1, X2),
@@ -1573,7 +1581,7 @@ __device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uin
  * columns magnifying one near texel) are merged by a segmented reduction
  * across the wave, so that the LDS table below sees distinct texels per
  * instruction; then an LDS hash table sums the terms per texel over the
- * whole 64 x 32 block (neighbouring samples share texels: a block touches
+ * whole 64 x 16 block (neighbouring samples share texels: a block touches
  * about a seventh as many texels as it has sample-texel terms).  Short
  * per-thread row runs keep many waves in flight for the gathers.  Each
  * table entry is then added into the global f32 gradient with one atomic
@@ -1631,27 +1639,8 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
 #pragma unroll
             for (int t = 0; t < 4; t++)
                 val[t][0] = val[t][1] = val[t][2] = 0.0f;
-            if (flush) {
-                Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
-                float sv[3][4], px[3];
-                sample_colour(a, b, plane, sv, px);
-                const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
-                const float wx0n = b.wx0 / wxs, wx1n = b.wx1 / wxs;
-                const float wy0n = b.wy0 / wys, wy1n = b.wy1 / wys;
-                const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
-                float q[3];
-#pragma unroll
-                for (int ch = 0; ch < 3; ch++)
-                    q[ch] = px[ch] > 0.0f ? run[ch] / px[ch] : 0.0f;
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const float sf = a.sf[b.t[t]];
-                    key[t] = (uint32_t)b.t[t];
-#pragma unroll
-                    for (int ch = 0; ch < 3; ch++)
-                        val[t][ch] = ((q[ch] * wt[t]) * sv[ch][t]) * sf;
-                }
-            }
+            if (flush)
+                sample_terms<P2>(a, k, col, plane, run, key, val);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const bool head = segment_sum3(key[t], val[t]);
@@ -1695,6 +1684,37 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
     }
 }
 
+/* L2 loss of the fitting harness: g = 2 * (img - target) and one partial
+ * sum((img - target)^2) per workgroup. */
+__global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
+                                                 const float *__restrict__ tgt, size_t n,
+                                                 float *__restrict__ g, float *__restrict__ part)
+{
+    __shared__ float wpart[4];
+    float acc = 0.0f;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t n4 = n / 4;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        float4 a = ((const float4 *)img)[i], b = ((const float4 *)tgt)[i];
+        float4 d = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+        ((float4 *)g)[i] = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
+        acc += ((d.x * d.x) + (d.y * d.y)) + ((d.z * d.z) + (d.w * d.w));
+    }
+    for (size_t i = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float d = img[i] - tgt[i];
+        g[i] = 2.0f * d;
+        acc += d * d;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0)
+        wpart[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = (wpart[0] + wpart[1]) + (wpart[2] + wpart[3]);
+}
+
 /* Sum of n per-workgroup partials in a fixed order (one workgroup),
  * stored into *dst or added to it. */
 __global__ __launch_bounds__(1024) void k_sum_partials(const float *__restrict__ part, int32_t n,
@@ -1727,37 +1747,6 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const float *__restrict__
             t += wsum[w];
         *dst = accumulate ? *dst + t : t;
     }
-}
-
-/* L2 loss of the fitting harness: g = 2 * (img - target) and one partial
- * sum((img - target)^2) per workgroup. */
-__global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
-                                                 const float *__restrict__ tgt, size_t n,
-                                                 float *__restrict__ g, float *__restrict__ part)
-{
-    __shared__ float wpart[4];
-    float acc = 0.0f;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t n4 = n / 4;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        float4 a = ((const float4 *)img)[i], b = ((const float4 *)tgt)[i];
-        float4 d = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
-        ((float4 *)g)[i] = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
-        acc += ((d.x * d.x) + (d.y * d.y)) + ((d.z * d.z) + (d.w * d.w));
-    }
-    for (size_t i = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float d = img[i] - tgt[i];
-        g[i] = 2.0f * d;
-        acc += d * d;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        acc += __shfl_xor(acc, off);
-    if ((threadIdx.x & 63) == 0)
-        wpart[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        part[blockIdx.x] = (wpart[0] + wpart[1]) + (wpart[2] + wpart[3]);
 }
 
 /* A frame of one colour (the degenerate frames of the state's #math mode). */
@@ -1990,10 +1979,8 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
     dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
     if ((int64_t)grid.x * grid.y > n_partials)
         return hipErrorInvalidValue;
-    if (pow2(hc) && pow2(wc))
-        hipLaunchKernelGGL(k_diff_grad<true>, grid, dim3(kGradThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_diff_grad<false>, grid, dim3(kGradThreads), 0, s, a);
+    void (*kern)(DiffArgs) = pow2(hc) && pow2(wc) ? k_diff_grad<true> : k_diff_grad<false>;
+    hipLaunchKernelGGL(kern, grid, dim3(kGradThreads), 0, s, a);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials,
                        (int32_t)(grid.x * grid.y), d_gsky, sky_accumulate);
     return hipGetLastError();

@@ -266,3 +266,35 @@ def test_fused_step_accumulates(ctx):
     assert torch.allclose(g, tot_g, rtol=1e-5, atol=1e-6)
     assert abs(float(gs[0]) - float(tot_gs[0])) <= 1e-5 * max(abs(float(tot_gs[0])), 1.0)
     assert abs(float(loss[0]) - tot_loss) <= 1e-5 * tot_loss
+
+
+@pytest.mark.parametrize("size", [99, 100])
+def test_fused_step_odd_plane_sizes(ctx, size):
+    """Planes that are not powers of two (floor-mod wrap), and 3*99*99
+    values (not a multiple of 4: the gradient clear takes its own launch);
+    a view wide enough to wrap around the map."""
+    col, hts = scene.scene_maps(17, 128)
+    col = np.ascontiguousarray(col[:size, :size])
+    hts = np.ascontiguousarray(hts[:size, :size])
+    m = ctx.map(col, hts, shade_size=size, sun_height=SUN_H, sun_ang=SUN_A)
+    r = DiffRenderer(ctx, m)
+    alt = np.asarray(hts, dtype=np.int32) & 255
+    sf = O.shade_factor(O.shadow_counts(alt, O.sun(SUN_H, SUN_A), size))
+    v = View(x=50.5, y=20.25, height=240.0, angle=2.0, horizon=80.0, distance=400.0, fov=1.4,
+             h=72, w=150, d=0.004, z0=0.6, scale=90.0)
+    planes = planes_of(col)
+    tp = torch.from_numpy(planes).cuda()
+    target = torch.from_numpy(np.random.default_rng(size).random((3, v.h, v.w),
+                                                                 dtype=np.float32)).cuda()
+    gp = torch.full_like(tp, 9.0)
+    img, src, gp, gs, loss = r.l2_step(tp, 0.2, v, target, gplanes=gp,
+                                       gsky=torch.zeros(1, device="cuda"),
+                                       loss=torch.zeros(1, device="cuda"))
+    torch.cuda.synchronize()
+    want_img, want_src = O.render_f32(v, planes, sf, alt, 0.2)
+    assert np.array_equal(img.cpu().numpy(), want_img)
+    d = (want_img.astype(np.float64) - target.cpu().numpy())
+    assert abs(float(loss[0]) - float((d * d).sum())) <= 1e-5 * float((d * d).sum())
+    g_np = (2.0 * (img - target)).cpu().numpy()
+    want_gp, _ = O.render_f32_vjp(v, planes, sf, want_src, g_np)
+    assert _rel_err(gp.cpu().numpy(), want_gp) <= 1e-5

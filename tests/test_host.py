@@ -136,3 +136,98 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+class FskFrame(ctypes.Structure):
+    """fsk_frame of futspace_amd/csrc/fs_internal.h (host-side layout)."""
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("cam_x", "cam_y", "cam_h", "horizon", "Lx", "Ly", "Rx", "Ry", "d", "z0",
+                 "scale", "fw")] + [("n", ctypes.c_int32), ("rows", ctypes.c_int32),
+                                    ("cols", ctypes.c_int32), ("sky", ctypes.c_uint32),
+                                    ("tame", ctypes.c_int32)]
+
+
+def _tame_fn():
+    lib = ctypes.CDLL(os.path.join(ROOT, "futspace_amd", "libfutspace.so"))
+    fn = lib.fsk_frame_tame
+    fn.restype = ctypes.c_int32
+    fn.argtypes = [ctypes.POINTER(FskFrame)]
+    return fn
+
+
+def _frame(v):
+    """The host's frame_of_view in f32 (glibc sinf/cosf differ from numpy's
+    by at most an ulp, which the predicate's margins absorb)."""
+    f32 = np.float32
+    s, c, fov = f32(np.sin(f32(v.angle))), f32(np.cos(f32(v.angle))), f32(v.fov)
+    f = FskFrame()
+    f.cam_x, f.cam_y, f.cam_h, f.horizon = v.x, v.y, v.height, v.horizon
+    f.Lx, f.Ly = (-c) - s * fov, s - c * fov
+    f.Rx, f.Ry = c - s * fov, (-s) - c * fov
+    f.d, f.z0 = v.d, v.z0
+    f.scale = v.scale if v.scale > 0 else float(v.w // 2)
+    f.fw = float(v.w)
+    f.n = v.n_slices if v.n_slices > 0 else scene._zs_count(v.distance, v.d, v.z0)
+    f.rows, f.cols = v.h, v.w
+    return f
+
+
+def _extremes(f):
+    """max |x|, |y| over every sample the kernel evaluates (columns up to
+    w + 7) and max |rel| over heights 0..65535, in f32 as the kernel forms
+    them."""
+    f32 = np.float32
+    i = np.arange(1, f.n + 1, dtype=np.float32)
+    z = (i / f32(2)) * ((f32(2) * f32(f.z0)) + ((i - f32(1)) * f32(f.d)))
+    lx, ly, rx, ry = f32(f.Lx) * z, f32(f.Ly) * z, f32(f.Rx) * z, f32(f.Ry) * z
+    dx, dy = (rx - lx) / f32(f.fw), (ry - ly) / f32(f.fw)
+    sx, sy = lx + f32(f.cam_x), ly + f32(f.cam_y)
+    col = np.arange(0, int(f.fw) + 8, dtype=np.float32)[None, :]
+    x = sx[:, None] + col * dx[:, None]
+    y = sy[:, None] + col * dy[:, None]
+    iz = (f32(1) / z) * f32(f.scale)
+    rel = [((f32(f.cam_h) - f32(h)) * iz) + f32(f.horizon) for h in (0.0, 65535.0)]
+    return (float(np.abs(x).max()), float(np.abs(y).max()),
+            float(max(np.abs(r).max() for r in rel)))
+
+
+def test_tame_frames_of_the_workloads():
+    tame = _tame_fn()
+    _, hts = scene.plain_2048_maps()
+    for v in scene.plain_2048_views(hts)[::10]:
+        assert tame(ctypes.byref(_frame(v))) == 1
+    assert tame(ctypes.byref(_frame(scene.plain_1024_view()))) == 1
+
+
+def test_untame_frames():
+    tame = _tame_fn()
+    base = View(x=512.0, y=512.0, height=180.0, angle=0.3, horizon=150.0, distance=1000.0,
+                fov=1.2, h=600, w=800, d=0.001, z0=0.465, scale=240.0)
+    assert tame(ctypes.byref(_frame(base))) == 1
+    for change in (dict(z0=0.0), dict(d=-1e-4), dict(x=float("nan")), dict(height=float("inf")),
+                   dict(x=5e6), dict(y=-5e6), dict(horizon=2e9), dict(scale=1e9),
+                   dict(z0=1e-7)):
+        f = _frame(View(**{**base.__dict__, **change}))
+        assert tame(ctypes.byref(f)) == 0, change
+
+
+def test_tame_bound_holds_by_brute_force():
+    """Random frames near the bounds: whenever the predicate says tame,
+    every sample coordinate is below 2^22 and every row below 2^30 in
+    magnitude, evaluated in f32 over all slices and columns."""
+    tame = _tame_fn()
+    rng = np.random.default_rng(31)
+    accepted = 0
+    for _ in range(120):
+        v = View(x=float(rng.choice([1.0, -1.0]) * rng.uniform(3.0e6, 4.3e6)),
+                 y=float(rng.uniform(-4e6, 4e6)), height=float(rng.uniform(-1e4, 1e4)),
+                 angle=float(rng.uniform(0, 6.3)), horizon=float(rng.uniform(-1e8, 1e8)),
+                 distance=1000.0, fov=float(rng.uniform(0.5, 1.5)), h=64, w=int(rng.integers(1, 64)),
+                 d=float(rng.uniform(0.0, 0.05)), z0=float(10 ** rng.uniform(-3, 1)),
+                 scale=float(10 ** rng.uniform(0, 3.5)), n_slices=int(rng.integers(1, 300)))
+        f = _frame(v)
+        if tame(ctypes.byref(f)):
+            accepted += 1
+            mx, my, mr = _extremes(f)
+            assert mx < 2.0 ** 22 and my < 2.0 ** 22 and mr < 2.0 ** 30, (v, mx, my, mr)
+    assert accepted >= 10
