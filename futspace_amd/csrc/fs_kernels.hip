@@ -397,8 +397,8 @@ constexpr int kShortSpan = 24;   /* span rows a lane paints itself */
 constexpr int kGather = 2;       /* height gathers a lane keeps in flight */
 constexpr int kColourBatch = 1;  /* visible samples a lane colours per pass */
 
-/* Dynamic LDS of a tile render: slice table (float4 + inv_z per skewed
- * entry), the TC x rows colour stripe, the visible-sample queue and span
+/* Dynamic LDS of a tile render: slice table (float4 + inv_z per slot),
+ * the TC x rows colour stripe, the visible-sample queue and span
  * bottoms (at most min(rows, chunk) samples per column and chunk) and the
  * long-span list. */
 template <int TC>
@@ -415,11 +415,12 @@ size_t render_lds_bytes_tc(int32_t rows, int32_t chunk)
  * chunk by chunk:
  *   0. the chunk's slice table (position, step, inv_z) into LDS;
  *   1. lane (slice group g, column c) projects its kr consecutive slices;
- *   2. prefix-min over slice groups (wave scan, then a scan of the wave
- *      totals) gives the y-buffer in front of each lane's slices;
+ *   2. prefix-min over slice groups (wave scan, then each lane takes the
+ *      minimum of the earlier waves' totals and the chunk's carry-in)
+ *      gives the y-buffer in front of each lane's slices;
  *   3. samples strictly above it are visible (scan occlude + scatter,
- *      fut/voxel_renderer.fut:231, :244): they are queued and their rows
- *      marked in a per-column bitmask;
+ *      fut/voxel_renderer.fut:231, :244): they are queued with the bottom
+ *      of their span;
  *   4. each queued sample gets its bilinear argb colour and paints its span
  *      down to the next visible row below (the reference's top-down
  *      `scan fill_vline`, :246-249, seen from the sample's side).
@@ -430,8 +431,8 @@ size_t render_lds_bytes_tc(int32_t rows, int32_t chunk)
 template <int TC>
 struct TileShared {
     float sq[256];                 /* squares table, filled once per workgroup */
-    int32_t wtot[kMaxWaves][TC];
-    int32_t wpre[kMaxWaves][TC];
+    int32_t wtot[kMaxWaves][TC];   /* wave totals of the prefix-min */
+    int32_t wpre[kMaxWaves][TC];   /* wave prefixes of the transparent fill */
     int32_t carry[TC];
     uint32_t qn, nlong, transparent;
 };
@@ -1582,10 +1583,10 @@ __device__ __forceinline__ void sample_terms(const DiffArgs &a, int32_t k, int32
  * across the wave, so that the LDS table below sees distinct texels per
  * instruction; then an LDS hash table sums the terms per texel over the
  * whole 64 x 16 block (neighbouring samples share texels: a block touches
- * about a seventh as many texels as it has sample-texel terms).  Short
+ * several times fewer texels than it has sample-texel terms).  Short
  * per-thread row runs keep many waves in flight for the gathers.  Each
  * table entry is then added into the global f32 gradient with one atomic
- * per channel.  The sky term is reduced per wave. */
+ * per channel.  The sky term is summed per workgroup into one partial. */
 template <bool P2>
 __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
 {
