@@ -118,6 +118,7 @@ FUTSPACE_SIGNATURES = {
     "fsx_render_f32_l2_step": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, P, I32, I32, F32,
                                               P, I32, P, P, P, P, P, P]),
     "fsx_render_stateless": (ctypes.c_int, [P, P, P, I32, I32, ctypes.POINTER(FsxView), P]),
+    "fsx_render2": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, I64]),
     "fsx_context_device": (ctypes.c_int, [P]),
     "fsx_version": (ctypes.c_char_p, []),
     "fsx_sky": (U32, [F32]),

@@ -56,6 +56,8 @@ struct futhark_context {
     int prof_n;
     hipEvent_t *prof_ev;      /* FS_PROF_MAX pairs */
     float *d_partials;        /* per-workgroup partial sums of the reductions */
+    float *d_trig;            /* render2's per-column cos / sin */
+    size_t trig_cap;          /* floats in d_trig */
 };
 
 #define FS_PARTIALS 16384     /* floats in d_partials */
@@ -279,6 +281,8 @@ void futhark_context_free(struct futhark_context *ctx)
     }
     if (ctx->d_partials)
         hipFree(ctx->d_partials);
+    if (ctx->d_trig)
+        hipFree(ctx->d_trig);
     if (ctx->own_stream)
         hipStreamDestroy(ctx->own_stream);
     free(ctx->error);
@@ -1400,6 +1404,77 @@ int fsx_render(struct futhark_context *ctx, const struct fsx_map *map,
         return 1;
     }
     return render_into(ctx, map->t, map->s, view, dev_out, pitch);
+}
+
+/* render2 (fut/voxel_renderer.fut:257-305), the reference's ray-march
+ * renderer, over the map's masked heights and unshaded colours.  The
+ * per-column cos / sin of angle + horz_ang are computed here with the C
+ * library, as `futhark c` computes them, and copied to the device after
+ * the context stream has drained (the buffer is reused between calls). */
+int fsx_render2(struct futhark_context *ctx, const struct fsx_map *map,
+                const struct fsx_view *view, uint32_t *dev_out, int64_t pitch)
+{
+    if (!ctx || !map || !view)
+        return 1;
+    if (view->h < 0 || view->w < 0 || (pitch != 0 && pitch < view->w)) {
+        set_error(ctx, "fsx_render2: bad frame %d x %d (pitch %lld)", view->h, view->w,
+                  (long long)pitch);
+        return 1;
+    }
+    if (view->h == 0 || view->w == 0)
+        return 0;
+    if (!dev_out) {
+        set_error(ctx, "fsx_render2: null output");
+        return 1;
+    }
+    const int32_t n = i32_f32(view->distance / 4.0f);
+    const int32_t miss = i32_f32(view->distance + 1.0f);
+    /* the march stops at the first hit, which is the reference's result
+     * while a miss (distance i32(distance + 1)) cannot beat a hit */
+    if (n > (1 << 16) || (n >= 0 && miss <= n)) {
+        set_error(ctx, "fsx_render2: distance %g outside the supported range (distance / 4 "
+                       "at most 2^16)", (double)view->distance);
+        return 1;
+    }
+    const size_t need = (size_t)2 * (size_t)view->w;
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (need > ctx->trig_cap) {
+        if (ctx->d_trig)
+            HIPCHECK(ctx, hipFree(ctx->d_trig));
+        ctx->d_trig = NULL;
+        ctx->trig_cap = 0;
+        HIPCHECK(ctx, hipMalloc((void **)&ctx->d_trig, sizeof(float) * need));
+        ctx->trig_cap = need;
+    }
+    float *h = (float *)malloc(sizeof(float) * need);
+    if (!h) {
+        set_error(ctx, "fsx_render2: out of host memory");
+        return 1;
+    }
+    for (int32_t x = 0; x < view->w; x++) {
+        const float horz_ang = ((float)x / (float)view->w) * (2.5f - 1.0f);
+        h[x] = cosf(view->angle + horz_ang);
+        h[view->w + x] = sinf(view->angle + horz_ang);
+    }
+    hipError_t e = hipMemcpy(ctx->d_trig, h, sizeof(float) * need, hipMemcpyHostToDevice);
+    free(h);
+    HIPCHECK(ctx, e);
+    fsk_frame f;
+    frame_of_view(view, &f);
+    fsk_map m;
+    m.hq = map->t->d_hq;
+    m.hrows = map->t->rows;
+    m.hcols = map->t->cols;
+    m.hbits = map->t->hbits;
+    m.cq = NULL;
+    m.crows = 0;
+    m.ccols = 0;
+    int pe = prof_begin(ctx);
+    HIPCHECK(ctx, fsk_render2(&m, &f, ctx->d_trig, ctx->d_trig + view->w, map->t->d_color, n,
+                              dev_out, pitch > 0 ? pitch : view->w, ctx->stream));
+    prof_end(ctx, pe);
+    return 0;
 }
 
 int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,

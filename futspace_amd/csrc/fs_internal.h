@@ -44,6 +44,12 @@ typedef struct {
  * projected row below 2^30 in magnitude for any 16-bit height. */
 int32_t fsk_frame_tame(const fsk_frame *f);
 
+/* render2 (ray march) of one frame into out[rows][pitch]: d_cs/d_sn[cols]
+ * the host's cos / sin of angle + horz_ang per column, d_color the map's
+ * unshaded colours [hrows][hcols], n = i32(distance / 4). */
+int fsk_render2(const fsk_map *m, const fsk_frame *f, const float *d_cs, const float *d_sn,
+                const uint32_t *d_color, int32_t n, uint32_t *d_out, int64_t pitch, hipStream_t s);
+
 /* Plain render of one frame into out[rows][pitch]; the kernel builds its
  * own slice table per depth chunk. */
 int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,

@@ -881,6 +881,63 @@ __global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
                                                      lds, sh);
 }
 
+/* render2 (fut/voxel_renderer.fut:257-305), the reference's ray-march
+ * renderer, one thread per pixel.  The reference evaluates every
+ * dist = 0 ... n and keeps the hit with the smallest dist (redstuff,
+ * :254-255); dists grow along the ray, so that is the first hit, and the
+ * march stops there (hits are the only elements that can win while the
+ * miss distance i32(distance + 1) exceeds n, which the host checks).  The
+ * colour is bilinear over the map's unshaded colours, evaluated for the
+ * hit only.  cs/sn hold cos and sin of angle + horz_ang per column,
+ * computed on the host with the C library as `futhark c` does. */
+struct Render2Args {
+    fsk_map m;
+    const uint32_t *color;     /* unshaded colours [hrows][hcols] */
+    const float *cs, *sn;      /* [cols] */
+    uint32_t *out;
+    int64_t pitch;
+    float cam_x, cam_y, cam_h, horizon;
+    int32_t n, rows, cols;
+};
+
+template <bool P2>
+__device__ __forceinline__ uint4 raw_color_quad(const uint32_t *col, int32_t rows, int32_t cols,
+                                                float x, float y)
+{
+    const uint32_t fx = (uint32_t)wrap<P2>(idx_of(floorf(x)), cols);
+    const uint32_t cx = (uint32_t)wrap<P2>(idx_of(ceilf(x)), cols);
+    const uint32_t fy = (uint32_t)wrap<P2>(idx_of(floorf(y)), rows);
+    const uint32_t cy = (uint32_t)wrap<P2>(idx_of(ceilf(y)), rows);
+    const uint32_t w = (uint32_t)cols;
+    return make_uint4(col[fy * w + fx], col[fy * w + cx], col[cy * w + fx], col[cy * w + cx]);
+}
+
+template <int HBITS, bool HP2>
+__global__ __launch_bounds__(256) void k_render2(Render2Args a)
+{
+    __shared__ float sq[256];
+    fill_square_lut(sq, threadIdx.x);
+    __syncthreads();
+    const int32_t x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int32_t y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.cols || y >= a.rows)
+        return;
+    const float vert_ang = (a.horizon / 50.0f) + (((float)y / ((float)a.rows - 0.0f)) * (1.5f - 1.0f));
+    const float ca = a.cs[x], sa = a.sn[x];
+    uint32_t c = 0u;
+    for (int32_t dist = 0; dist <= a.n; dist++) {
+        const float fd = (float)dist;
+        const float px = a.cam_x + (fd * ca), py = a.cam_y + (fd * sa);
+        const float hgt = height_interp<HBITS>(height_quad<HBITS, HP2>(a.m, px, py), px, py);
+        if ((a.cam_h - (fd * vert_ang)) < hgt) {
+            c = color_of_quad(raw_color_quad<HP2>(a.color, a.m.hrows, a.m.hcols, px, py), sq, px,
+                              py);
+            break;
+        }
+    }
+    a.out[(size_t)y * (size_t)a.pitch + (size_t)x] = c;
+}
+
 /* ------------------------------------------------------------------ */
 /* Map preparation kernels                                             */
 /* ------------------------------------------------------------------ */
@@ -2040,6 +2097,33 @@ int fsk_shade_factor(const uint16_t *d_hplain, int32_t hrows, int32_t hcols, flo
     dim3 grid((size + 255) / 256, size);
     hipLaunchKernelGGL(k_shade_factor, grid, dim3(256), 0, s, d_hplain, hrows, hcols, sun0, sun1,
                        sun2, size, d_sf);
+    return hipGetLastError();
+}
+
+int fsk_render2(const fsk_map *m, const fsk_frame *f, const float *d_cs, const float *d_sn,
+                const uint32_t *d_color, int32_t n, uint32_t *d_out, int64_t pitch, hipStream_t s)
+{
+    if (f->rows <= 0 || f->cols <= 0)
+        return hipSuccess;
+    Render2Args a;
+    a.m = *m;
+    a.color = d_color;
+    a.cs = d_cs;
+    a.sn = d_sn;
+    a.out = d_out;
+    a.pitch = pitch;
+    a.cam_x = f->cam_x;
+    a.cam_y = f->cam_y;
+    a.cam_h = f->cam_h;
+    a.horizon = f->horizon;
+    a.n = n;
+    a.rows = f->rows;
+    a.cols = f->cols;
+    dim3 grid((f->cols + 63) / 64, (f->rows + 3) / 4);
+    const bool p2 = pow2(m->hrows) && pow2(m->hcols);
+    void (*kern)(Render2Args) = m->hbits == 8 ? (p2 ? k_render2<8, true> : k_render2<8, false>)
+                                              : (p2 ? k_render2<16, true> : k_render2<16, false>);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

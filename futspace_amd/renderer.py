@@ -133,6 +133,26 @@ class Context:
                                        ctypes.c_void_p(out.data_ptr()), pitch), "fsx_render")
         return out
 
+    def render2(self, m, view: View, out=None):
+        """render2 (fsx_render2): the reference's ray-march renderer over
+        the map's heights and unshaded colours.  out=None returns a host
+        numpy uint32 [h][w] frame; a torch CUDA int32/uint32 [h][>=w] tensor
+        is written in place, asynchronously on the context stream."""
+        import torch
+        own = out is None
+        if own:
+            out = torch.empty((view.h, view.w), dtype=torch.int32, device="cuda")
+        if out.dim() != 2 or out.shape[0] != view.h or out.shape[1] < view.w or out.stride(1) != 1:
+            raise ValueError(f"output {tuple(out.shape)} does not hold a {view.h}x{view.w} frame")
+        cv = view.to_c()
+        self.check(self.lib.fsx_render2(self.ptr, m.ptr, ctypes.byref(cv),
+                                        ctypes.c_void_p(out.data_ptr()), out.stride(0)),
+                   "fsx_render2")
+        if own:
+            self.sync()
+            return out.cpu().numpy().view(np.uint32)
+        return out
+
     def render_batch(self, maps, views, out, offsets):
         """One batched launch: views[i] into out.flat[offsets[i]:...]."""
         n = len(views)

@@ -159,6 +159,18 @@ int fsx_render(struct futhark_context *ctx, const struct fsx_map *map,
 int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,
                     const struct fsx_view *view, uint32_t *host_out);
 
+/* render2 (fut/voxel_renderer.fut:257-305), the reference's ray-march
+ * renderer, which no generated entry point calls: pixel (y, x) marches
+ * dist = 0 ... i32(distance / 4) along angle + (x / w) * 1.5 and takes the
+ * bilinear unshaded colour of the first point whose bilinear height is
+ * above height - dist * (horizon / 50 + (y / h) * 0.5), else 0.  Uses the
+ * view's x, y, height, angle, horizon, distance and frame size; distance / 4
+ * at most 2^16 (the march's length per pixel).  dev_out [h][w] u32 (row pitch in elements, 0 = w),
+ * asynchronous on the context stream after a short synchronous upload of
+ * the per-column cos / sin. */
+int fsx_render2(struct futhark_context *ctx, const struct fsx_map *map,
+                const struct fsx_view *view, uint32_t *dev_out, int64_t pitch);
+
 /* Batched render: views[i] draws maps[views[i].map] into dev_out +
  * offsets[i] (u32 elements, row pitch views[i].w).  One persistent
  * launch covers all views; per-XCD work queues hand out column tiles. */

@@ -221,6 +221,51 @@ uint32_t fso_png_color_filtered(const uint32_t *col, int32_t h, int32_t w, float
     return fso_mix(ceil_y - y, x_interpolated1, y - floor_y, x_interpolated2);
 }
 
+/* render2, fut/voxel_renderer.fut:257-305: the ray-march renderer, which
+ * no entry point calls.  Pixel (y, x): vert_ang = horizon/50 +
+ * (y / (h - 0)) * (1.5 - 1.0) (:292), horz_ang = (x / w) * (2.5 - 1.0)
+ * (:295); for every dist in 0 ... i32(distance / 4), both ends included
+ * (:296-300), the point (x + dist cos(angle + horz_ang), y + dist
+ * sin(angle + horz_ang)) holds the bilinear height and colour of the
+ * landscape's altitude and (unshaded) colour maps (render_map, :262-290);
+ * it is (colour, dist) when height - dist * vert_ang < that height, else
+ * (0, i32(distance + 1)); the pixel is the reduce with redstuff
+ * (:254-255) from (0, 100000).  Restated literally: every dist is
+ * evaluated, colour included, and folded in order. */
+int fso_render2(const fso_camera *cam, const uint32_t *col, const int32_t *alt, int32_t r,
+                int32_t s, int32_t h, int32_t w, uint32_t *out)
+{
+    if (r <= 0 || s <= 0 || h < 0 || w < 0)
+        return 1;
+    const int32_t n = fso_i32_f32(cam->distance / 4.0f);
+    const int32_t miss = fso_i32_f32(cam->distance + 1.0f);
+    for (int32_t y = 0; y < h; y++) {
+        const float vert_ang = (cam->horizon / 50.0f) + (((float)y / ((float)h - 0.0f)) * (1.5f - 1.0f));
+        for (int32_t x = 0; x < w; x++) {
+            const float horz_ang = ((float)x / (float)w) * (2.5f - 1.0f);
+            const float ca = cosf(cam->angle + horz_ang), sa = sinf(cam->angle + horz_ang);
+            uint32_t best_c = 0u;
+            int32_t best_d = 100000;
+            for (int32_t dist = 0; dist <= n; dist++) {
+                const float fd = (float)dist;
+                const float px = cam->x + (fd * ca), py = cam->y + (fd * sa);
+                const float height = fso_png_height_filtered(alt, r, s, px, py);
+                const uint32_t c = fso_png_color_filtered(col, r, s, px, py);
+                const int hit = (cam->height - (fd * vert_ang)) < height;
+                const uint32_t ec = hit ? c : 0u;
+                const int32_t ed = hit ? dist : miss;
+                /* redstuff acc elem: acc if acc.dist < elem.dist, else elem */
+                if (!(best_d < ed)) {
+                    best_c = ec;
+                    best_d = ed;
+                }
+            }
+            out[(size_t)y * (size_t)w + (size_t)x] = best_c;
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Renderer, #off branch                                               */
 /* ------------------------------------------------------------------ */

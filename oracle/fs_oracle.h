@@ -29,6 +29,9 @@
  *   fso_shadowmap         closed-form sun vector, flat map (count 0),
  *                         single spike counted by hand
  *   fso_render_nearest    painter cross-check (tests/futspace.fut)
+ *   fso_render2           camera inside terrain (uniform frame of the
+ *                         bilinear colour under it), flat-map hit rows in
+ *                         closed form (vert_ang, inclusive march end)
  *   fso_mix/from/to/scale PARITY UNPINNED: matte 0.1.2 (colour.fut) is
  *                         not vendored in the reference; the formula is
  *                         the Gloss mixColors restatement, checked only
@@ -86,6 +89,11 @@ uint32_t fso_png_color(const uint32_t *col, int32_t h, int32_t w, float x, float
 uint32_t fso_png_color_filtered(const uint32_t *col, int32_t h, int32_t w, float x, float y);
 
 /* ---- renderer, #off branch (fut/voxel_renderer.fut:214-251) ---- */
+/* render2, fut/voxel_renderer.fut:257-305 (ray march over the altitude and
+ * unshaded colour maps [r][s]) into out[h][w]. */
+int fso_render2(const fso_camera *cam, const uint32_t *col, const int32_t *alt, int32_t r,
+                int32_t s, int32_t h, int32_t w, uint32_t *out);
+
 int fso_render_off(const fso_camera *cam, const fso_depth *dp,
                    const uint32_t *col, int32_t hc, int32_t wc,
                    const int32_t *alt, int32_t ha, int32_t wa,

@@ -66,6 +66,7 @@ _SIG = {
     "fso_render_nearest_painter": (ctypes.c_int, [CP, P, P, I, I, I, I, I, P]),
     "fso_render_f32": (ctypes.c_int, [CP, DP, P, P, I, I, P, I, I, I, I, F, P, P]),
     "fso_render_f32_vjp": (ctypes.c_int, [CP, DP, P, P, I, I, I, I, P, P, P, P]),
+    "fso_render2": (ctypes.c_int, [CP, P, P, I, I, I, I, P]),
 }
 
 _lib = None
@@ -190,6 +191,18 @@ def math_shadowmap(sun_vec):
     s = _f32(sun_vec)
     out = np.empty((1024, 1024), dtype=np.uint32)
     assert lib().fso_math_shadowmap(s.ctypes.data, out.ctypes.data) == 0
+    return out
+
+
+def render2(view, color, alt):
+    """render2 (fut/voxel_renderer.fut:257-305): the ray march over the
+    altitude and unshaded colour maps; view supplies the camera and frame
+    size."""
+    c, a = _u32(color), _i32(alt)
+    out = np.empty((view.h, view.w), dtype=np.uint32)
+    rc = lib().fso_render2(ctypes.byref(camera(view)), c.ctypes.data, a.ctypes.data,
+                           a.shape[0], a.shape[1], view.h, view.w, out.ctypes.data)
+    assert rc == 0
     return out
 
 

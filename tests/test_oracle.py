@@ -434,3 +434,41 @@ def test_math_samplers():
     # flat regions of expo2 never shadow: a sun straight up counts nothing
     mm = O.math_shadowmap(O.sun(0.0, 0.0))
     assert set(np.unique(mm).tolist()) <= {0xFF000000, 0xFFFFFFFF}
+
+
+# ---- render2 (fut/voxel_renderer.fut:257-305) ----
+
+def test_render2_inside_terrain_is_uniform():
+    """A camera below the terrain hits at dist 0 in every pixel: the frame
+    is the bilinear colour under the camera everywhere."""
+    col, hts = scene.scene_maps(21, 64)
+    alt = np.asarray(hts, dtype=np.int32) & 255
+    v = View(x=20.37, y=41.61, height=-5.0, angle=1.1, horizon=30.0, distance=120.0, fov=1.2,
+             h=24, w=32)
+    img = O.render2(v, col, alt)
+    want = O.png_color_filtered(col, np.float32(v.x), np.float32(v.y))
+    assert np.all(img == want) and want != 0
+
+
+def test_render2_hits_on_a_flat_map():
+    """Heights 0 everywhere: a pixel is a hit iff some dist in 0 ... n has
+    dist * vert_ang > height (f32, vert_ang = horizon/50 + (y/h) * 0.5),
+    n = i32(distance / 4) included; hit pixels show the (non-zero) colour
+    unless the hit point is integral."""
+    rows = cols = 32
+    col = np.full((rows, cols), 0xFF3366CC, dtype=np.uint32)
+    alt = np.zeros((rows, cols), dtype=np.int32)
+    f32 = np.float32
+    for hor, height, dist in ((-20.0, 3.0, 101.0), (5.0, 7.5, 60.0), (-40.0, 1.25, 400.0)):
+        v = View(x=3.3, y=7.7, height=height, angle=0.35, horizon=hor, distance=dist, fov=1.2,
+                 h=48, w=40)
+        img = O.render2(v, col, alt)
+        n = int(f32(dist) / f32(4.0))
+        d = np.arange(n + 1, dtype=np.float32)
+        for y in range(v.h):
+            va = (f32(hor) / f32(50.0)) + ((f32(y) / f32(v.h)) * f32(0.5))
+            hit = bool(np.any((f32(height) - d * va) < f32(0.0)))
+            if hit:
+                assert np.all(img[y] != 0), (hor, y)
+            else:
+                assert np.all(img[y] == 0), (hor, y)
