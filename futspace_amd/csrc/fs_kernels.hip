@@ -393,7 +393,7 @@ __device__ __forceinline__ uint32_t color_of_quad(uint4 q, const float *sq, floa
     return out | (chan_u8(ay) << 24);
 }
 
-constexpr int kShortSpan = 24;   /* span rows a lane paints itself */
+constexpr int kShortSpan = 40;   /* span rows a lane paints itself */
 constexpr int kGather = 2;       /* height gathers a lane keeps in flight */
 constexpr int kColourBatch = 1;  /* visible samples a lane colours per pass */
 
@@ -497,7 +497,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     const int32_t qcap = TC * min(H, chunk_max);                 /* visible rows differ */
     uint32_t *queue = stripe + (size_t)Hp * TC;                  /* [qcap] */
     uint16_t *qbot = (uint16_t *)(queue + qcap);                 /* [qcap] span bottoms */
-    uint32_t *longs = queue + qcap + (qcap + 1) / 2;             /* [2][TC*(H/24+1)] */
+    uint32_t *longs = queue + qcap + (qcap + 1) / 2;             /* [2][TC*(H/kShortSpan+1)] */
 
     const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t col0 = tile * TC;
