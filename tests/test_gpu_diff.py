@@ -298,3 +298,19 @@ def test_fused_step_odd_plane_sizes(ctx, size):
     g_np = (2.0 * (img - target)).cpu().numpy()
     want_gp, _ = O.render_f32_vjp(v, planes, sf, want_src, g_np)
     assert _rel_err(gp.cpu().numpy(), want_gp) <= 1e-5
+
+
+def test_forward_tall_frame(ctx):
+    """The differentiable forward on a frame tall enough for 4-column
+    slice-mode tiles."""
+    r, col, alt, sf = _setup(ctx)
+    v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=900.0, distance=300.0, fov=1.2,
+             h=2200, w=40, d=0.01, z0=0.5, scale=700.0)
+    planes = planes_of(col)
+    img, src = r.forward_raw(torch.from_numpy(planes).cuda(), 0.37, v)
+    torch.cuda.synchronize()
+    want_img, want_src = O.render_f32(v, planes, sf, alt, 0.37)
+    got_src = src.cpu().numpy()
+    cols = np.arange(v.w, dtype=np.int32)[None, :]
+    assert np.array_equal(np.where(got_src < 0, -1, got_src * v.w + cols), want_src)
+    assert np.array_equal(img.cpu().numpy(), want_img)

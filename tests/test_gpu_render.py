@@ -205,3 +205,24 @@ def test_kernel_shapes_bitexact(ctx, small_scene, shape):
         _check(ctx.render(m, v), O.render_off(v, shaded, alt))
     finally:
         ctx.set_tuning(0, 0, 0)
+
+
+@pytest.mark.parametrize("h,w", [(2160, 72), (4096, 24)])
+def test_tall_frames(ctx, small_scene, h, w):
+    """Frames whose 8-column stripe does not fit in LDS take 4-column
+    tiles (and, at 4096 rows, the shortest chunks): still bit-exact."""
+    col, hts = small_scene
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    v = View(x=40.3, y=90.6, height=230.0, angle=1.3, horizon=h * 0.4, distance=400.0, fov=1.2,
+             sky_color=sky_color(0.1), h=h, w=w, d=0.004, z0=0.5, scale=h / 3.0)
+    _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+
+
+def test_rows_beyond_the_limit_are_rejected(ctx, small_scene):
+    col, hts = small_scene
+    m = ctx.map(col, hts)
+    v = View(x=40.3, y=90.6, height=230.0, angle=1.3, horizon=100.0, distance=400.0, fov=1.2,
+             h=4097, w=8, d=0.004, z0=0.5)
+    with pytest.raises(RuntimeError):
+        ctx.render(m, v)
