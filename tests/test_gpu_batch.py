@@ -132,3 +132,20 @@ def test_autotuner_picks_a_candidate(ctx, pow2_maps, tmp_path, monkeypatch):
         assert again == best
     finally:
         ctx.set_tuning(0, 0, 0)
+
+
+def test_batch_with_a_tall_view(ctx, pow2_maps):
+    """One 2400-row view among small ones: the batch's shape follows its
+    tallest view (4-column tiles); every view still equals its single
+    render and the tall one equals the oracle."""
+    maps, shapes = pow2_maps
+    views = _views(12, shapes, seed=13, max_h=120)
+    views.append(View(x=100.25, y=60.5, height=300.0, angle=2.4, horizon=900.0, distance=500.0,
+                      fov=1.2, sky_color=0xFF9090E0, h=2400, w=40, d=0.003, z0=0.5,
+                      scale=800.0, map=0))
+    flat, offs = _run_batch(ctx, maps, views)
+    _check_views(ctx, maps, views, flat, offs)
+    col, hts = scene.scene_maps(3, 256)
+    shaded, alt = O.update_map(col, hts)
+    v, o = views[-1], offs[-1]
+    assert np.array_equal(flat[o:o + v.h * v.w].reshape(v.h, v.w), O.render_off(v, shaded, alt))
