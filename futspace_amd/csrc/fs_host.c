@@ -60,7 +60,7 @@ struct futhark_context {
     size_t trig_cap;          /* floats in d_trig */
 };
 
-#define FS_PARTIALS 16384     /* floats in d_partials */
+#define FS_PARTIALS 32768     /* floats in d_partials */
 
 #define FS_PROF_MAX 4096
 
@@ -1710,12 +1710,19 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
     /* forward colour, loss terms and (unless accumulating) the clear of the
      * gradient planes in one launch; the loss partials are summed before
      * the adjoint reuses the scratch */
+    /* the forward's loss and sky partials take the scratch's first and
+     * second quarter and are summed by the adjoint's first workgroup (the
+     * sky term of the loss gradient is known once g is); the adjoint's own
+     * partials would take the second half */
+    const int32_t q = FS_PARTIALS / 4;
+    int32_t nloss = 0;
     HIPCHECK(ctx, fsk_diff_color_l2(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_target,
                                     dev_img, dev_gimg, dev_loss, accumulate != 0,
-                                    accumulate ? NULL : dev_gplanes, nplanes, part, FS_PARTIALS,
-                                    ctx->stream));
-    HIPCHECK(ctx, fsk_diff_grad(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg, dev_gplanes,
-                                dev_gsky, accumulate != 0, part, FS_PARTIALS, ctx->stream));
+                                    accumulate ? NULL : dev_gplanes, nplanes, part, q, &nloss,
+                                    part + q, ctx->stream));
+    HIPCHECK(ctx, fsk_diff_grad_l2(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg, dev_gplanes,
+                                   dev_gsky, accumulate != 0, part + 2 * q, 2 * q, part, nloss,
+                                   dev_loss, accumulate != 0, part + q, ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }

@@ -1354,6 +1354,42 @@ hipError_t launch_batch(const fsk_batch *fb, hipStream_t s)
  * (argb.scale of 0xFF9090e0, fut/interactive.fut:163). */
 
 
+/* Sum of n per-workgroup partials in a fixed order by one workgroup of NT
+ * threads, stored into *dst or added to it: thread t adds part[t],
+ * part[t + NT], ... in that order (eight loads issued before the first
+ * add), then the wave totals are added in wave order.  Every thread of the
+ * workgroup must call it. */
+template <int NT>
+__device__ __forceinline__ void block_sum_partials(const float *__restrict__ part, int32_t n,
+                                                   float *__restrict__ dst, int32_t accumulate)
+{
+    __shared__ float wsum[NT / 64];
+    float acc = 0.0f;
+    for (int32_t base = 0; base < n; base += 8 * NT) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int32_t i = base + u * NT + (int32_t)threadIdx.x;
+            v[u] = i < n ? part[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            acc += v[u];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0)
+        wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < NT / 64; w++)
+            t += wsum[w];
+        *dst = accumulate ? *dst + t : t;
+    }
+}
+
 struct DiffArgs {
     fsk_frame f;
     const float *cp;
@@ -1366,6 +1402,11 @@ struct DiffArgs {
     float *gcp;            /* backward: dL/dcp accumulated [3][hc][wc] */
     float *gsky;           /* backward: dL/dsky_scale accumulated */
     float *partials;       /* per-workgroup partial sums (loss or sky) */
+    const float *loss_part;    /* backward of the fused step: the forward's */
+    int32_t loss_n;            /* loss and sky partials, summed by */
+    float *loss;               /* workgroup (0, 0) */
+    int32_t loss_accumulate;
+    float *sky_part;           /* the fused forward's per-workgroup sky terms */
 };
 
 __device__ __forceinline__ float mixf(float m1, float a, float m2, float b)
@@ -1487,7 +1528,7 @@ __global__ __launch_bounds__(1024) void k_diff_color_l2(DiffArgs a, const float 
     const int32_t col = (b % nbx) * 64 + (threadIdx.x & 63);
     const int32_t row = (b / nbx) * kStepRows + (threadIdx.x >> 6);
     const int32_t H = a.f.rows, W = a.f.cols;
-    float acc = 0.0f;
+    float acc = 0.0f, sky = 0.0f;
     if (col < W && row < H) {
         const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
         const size_t p = (size_t)row * W + col;
@@ -1507,20 +1548,31 @@ __global__ __launch_bounds__(1024) void k_diff_color_l2(DiffArgs a, const float 
             a.img[ch * fplane + p] = px[ch];
             g[ch * fplane + p] = 2.0f * d;
             acc += d * d;
+            /* d loss / d sky_scale of a sky pixel: g * (0x90, 0x90, 0xE0) / 255 */
+            if (k < 0)
+                sky += (2.0f * d) * sky_base(ch);
         }
     }
-    __shared__ float wpart[16];
+    __shared__ float wpart[16], wsky[16];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1) {
         acc += __shfl_xor(acc, off);
-    if ((threadIdx.x & 63) == 0)
+        sky += __shfl_xor(sky, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
         wpart[threadIdx.x >> 6] = acc;
+        wsky[threadIdx.x >> 6] = sky;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        float t = 0.0f;
-        for (int w = 0; w < 16; w++)
+        float t = 0.0f, ts = 0.0f;
+        for (int w = 0; w < 16; w++) {
             t += wpart[w];
+            ts += wsky[w];
+        }
         a.partials[b] = t;
+        if (a.sky_part)
+            a.sky_part[b] = ts;
     }
 }
 
@@ -1716,6 +1768,13 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
             run[0] = run[1] = run[2] = 0.0f;
         }
     }
+    /* the fused step's loss: its partials are complete (the forward's
+     * launch has ended), one workgroup sums them */
+    if (a.loss_n > 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+        block_sum_partials<kGradThreads>(a.loss_part, a.loss_n, a.loss, a.loss_accumulate);
+        __syncthreads();
+        block_sum_partials<kGradThreads>(a.sky_part, a.loss_n, a.gsky, a.loss_accumulate);
+    }
     __shared__ float wsky[kGradThreads / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -1723,7 +1782,8 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
     if ((threadIdx.x & 63) == 0)
         wsky[threadIdx.x >> 6] = sky_acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    /* the fused step took the sky term from its forward pass */
+    if (threadIdx.x == 0 && a.loss_n == 0) {
         float t = 0.0f;
         for (int w = 0; w < kGradThreads / 64; w++)
             t += wsky[w];
@@ -1773,38 +1833,10 @@ __global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
         part[blockIdx.x] = (wpart[0] + wpart[1]) + (wpart[2] + wpart[3]);
 }
 
-/* Sum of n per-workgroup partials in a fixed order (one workgroup),
- * stored into *dst or added to it. */
 __global__ __launch_bounds__(1024) void k_sum_partials(const float *__restrict__ part, int32_t n,
                                                        float *__restrict__ dst, int32_t accumulate)
 {
-    __shared__ float wsum[16];
-    float acc = 0.0f;
-    /* thread t adds part[t], part[t + 1024], ... in that order; eight
-     * loads are issued before the first add */
-    for (int32_t base = 0; base < n; base += 8 * 1024) {
-        float v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int32_t i = base + u * 1024 + (int32_t)threadIdx.x;
-            v[u] = i < n ? part[i] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-            acc += v[u];
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        acc += __shfl_xor(acc, off);
-    if ((threadIdx.x & 63) == 0)
-        wsum[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.0f;
-        for (int w = 0; w < 16; w++)
-            t += wsum[w];
-        *dst = accumulate ? *dst + t : t;
-    }
+    block_sum_partials<1024>(part, n, dst, accumulate);
 }
 
 /* A frame of one colour (the degenerate frames of the state's #math mode). */
@@ -1943,6 +1975,11 @@ static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, 
     a.gcp = nullptr;
     a.gsky = nullptr;
     a.partials = nullptr;
+    a.loss_part = nullptr;
+    a.loss_n = 0;
+    a.loss = nullptr;
+    a.loss_accumulate = 0;
+    a.sky_part = nullptr;
     return a;
 }
 
@@ -2027,6 +2064,16 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
                   float *d_gsky, int32_t sky_accumulate, float *d_partials, int32_t n_partials,
                   hipStream_t s)
 {
+    return fsk_diff_grad_l2(f, d_cp, d_sf, hc, wc, d_src, d_gimg, d_gcp, d_gsky, sky_accumulate,
+                            d_partials, n_partials, nullptr, 0, nullptr, 0, nullptr, s);
+}
+
+int fsk_diff_grad_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                     int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
+                     float *d_gsky, int32_t sky_accumulate, float *d_partials,
+                     int32_t n_partials, const float *d_loss_part, int32_t loss_n, float *d_loss,
+                     int32_t loss_accumulate, const float *d_sky_part, hipStream_t s)
+{
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
     DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
@@ -2034,13 +2081,19 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
     a.gcp = d_gcp;
     a.gsky = d_gsky;
     a.partials = d_partials;
+    a.loss_part = d_loss_part;
+    a.loss_n = loss_n;
+    a.loss = d_loss;
+    a.loss_accumulate = loss_accumulate;
+    a.sky_part = (float *)d_sky_part;
     dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
     if ((int64_t)grid.x * grid.y > n_partials)
         return hipErrorInvalidValue;
     void (*kern)(DiffArgs) = pow2(hc) && pow2(wc) ? k_diff_grad<true> : k_diff_grad<false>;
     hipLaunchKernelGGL(kern, grid, dim3(kGradThreads), 0, s, a);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials,
-                       (int32_t)(grid.x * grid.y), d_gsky, sky_accumulate);
+    if (loss_n == 0)
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials,
+                           (int32_t)(grid.x * grid.y), d_gsky, sky_accumulate);
     return hipGetLastError();
 }
 
@@ -2048,11 +2101,12 @@ int fsk_diff_color_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, 
                       int32_t wc, float sky_scale, const int32_t *d_src, const float *d_tgt,
                       float *d_img, float *d_g, float *d_loss, int32_t loss_accumulate,
                       float *d_zero, size_t zero_n, float *d_partials, int32_t n_partials,
-                      hipStream_t s)
+                      int32_t *n_loss_partials, float *d_sky_part, hipStream_t s)
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
     DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
+    a.sky_part = d_sky_part;
     a.sky_scale = sky_scale;
     a.img = d_img;
     a.partials = d_partials;
@@ -2086,8 +2140,11 @@ int fsk_diff_color_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, 
     else
         hipLaunchKernelGGL(k_diff_color_l2<false>, grid, dim3(1024), 0, s, a, d_tgt, d_g, nbx,
                            (int32_t)nzero, every, z, zn4);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials, (int32_t)npix,
-                       d_loss, loss_accumulate);
+    if (n_loss_partials)
+        *n_loss_partials = (int32_t)npix;    /* the caller sums them */
+    else
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials, (int32_t)npix,
+                           d_loss, loss_accumulate);
     return hipGetLastError();
 }
 
