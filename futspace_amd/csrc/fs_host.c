@@ -1706,10 +1706,12 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
     m.cq = NULL;
     m.crows = hc;
     m.ccols = wc;
-    HIPCHECK(ctx, fsk_render_slices(&m, &f, dev_src, view->w, ctx->stream));
-    /* forward colour, loss terms and (unless accumulating) the clear of the
-     * gradient planes in one launch; the loss partials are summed before
-     * the adjoint reuses the scratch */
+    /* unless accumulating, the gradient planes are cleared by the slice
+     * render's spare workgroups (CUs its tiles leave idle) or else inside
+     * the colour launch */
+    int32_t cleared = 0;
+    HIPCHECK(ctx, fsk_render_slices_clear(&m, &f, dev_src, view->w, accumulate ? NULL : dev_gplanes,
+                                          nplanes, &cleared, ctx->stream));
     /* the forward's loss and sky partials take the scratch's first and
      * second quarter and are summed by the adjoint's first workgroup (the
      * sky term of the loss gradient is known once g is); the adjoint's own
@@ -1718,7 +1720,8 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
     int32_t nloss = 0;
     HIPCHECK(ctx, fsk_diff_color_l2(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_target,
                                     dev_img, dev_gimg, dev_loss, accumulate != 0,
-                                    accumulate ? NULL : dev_gplanes, nplanes, part, q, &nloss,
+                                    accumulate || cleared ? NULL : dev_gplanes, nplanes, part, q,
+                                    &nloss,
                                     part + q, ctx->stream));
     HIPCHECK(ctx, fsk_diff_grad_l2(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg, dev_gplanes,
                                    dev_gsky, accumulate != 0, part + 2 * q, 2 * q, part, nloss,

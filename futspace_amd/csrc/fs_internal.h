@@ -120,6 +120,11 @@ int fsk_color_quads(const uint32_t *d_col, int32_t rows, int32_t cols, void *d_c
  * *d_gsky (both accumulate). */
 int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
                       hipStream_t s);
+/* fsk_render_slices that also clears d_zero[zero_n] (float4-aligned) in
+ * extra workgroups on the CUs its tiles leave idle, when enough are; sets
+ * *cleared to 1 if it did (else the caller clears). */
+int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
+                            float *d_zero, size_t zero_n, int32_t *cleared, hipStream_t s);
 /* The stateless program of tests/futspace.fut: m.hq = i32 heights and
  * m.cq = i32 colours, both [hrows][hcols], nearest sampling; f.sky is the
  * program's sky (also its `empty` colour). */

@@ -262,7 +262,17 @@ struct RenderArgs {
     int64_t pitch;
     int32_t ntiles;
     int32_t chunk_max;     /* slices per chunk, at most kChunk */
+    /* slice mode: workgroups ntiles .. ntiles + nzero - 1 clear zero[zero_n4]
+     * (the fused step's gradient) on CUs the tiles leave idle */
+    float4 *zero;
+    size_t zero_n4;
+    int32_t nzero;
 };
+
+/* Fewest idle workgroup slots worth a gradient clear inside the slice
+ * render: with fewer, each would clear too much of the buffer and
+ * outlast the tiles. */
+constexpr int32_t kMinZeroBlocks = 64;
 
 #ifndef FS_SQ_LUT
 #define FS_SQ_LUT 1
@@ -875,6 +885,15 @@ __global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
 {
     extern __shared__ uint32_t lds[];
     __shared__ TileShared<TC> sh;
+    if constexpr (MODE == kOutSlice) {
+        if ((int32_t)blockIdx.x >= a.ntiles) {
+            const size_t stride = (size_t)a.nzero * NT;
+            for (size_t i = (size_t)(blockIdx.x - a.ntiles) * NT + threadIdx.x; i < a.zero_n4;
+                 i += stride)
+                a.zero[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return;
+        }
+    }
     fill_square_lut(sh.sq, threadIdx.x);
     render_tile<TC, NT, CMAX, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
                                                      xcd_tile(blockIdx.x, a.ntiles), a.chunk_max,
@@ -1080,7 +1099,7 @@ hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
     size_t need = render_lds_bytes_tc<TC>(a.f.rows, a.chunk_max);
     if (need > (size_t)dyn_max)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(kern, dim3(a.ntiles), dim3(NT), need, s, a);
+    hipLaunchKernelGGL(kern, dim3(a.ntiles + a.nzero), dim3(NT), need, s, a);
     return hipGetLastError();
 }
 
@@ -1194,16 +1213,21 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
 }
 
 template <int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
-hipError_t launch_render(RenderArgs a, hipStream_t s)
+hipError_t launch_render(RenderArgs a, hipStream_t s, int32_t *nzero = nullptr)
 {
     Shape sh = resolve_shape(a.f.rows, current_tuning());
     a.chunk_max = sh.chunk;
+    a.ntiles = (a.f.cols + sh.tc - 1) / sh.tc;
+    /* slice mode: the workgroup slots the tiles leave free clear the
+     * gradient, if there are enough of them */
+    const int32_t spare = cu_count() * sh.wg - a.ntiles;
+    a.nzero = (MODE == kOutSlice && a.zero && a.zero_n4 && spare >= kMinZeroBlocks) ? spare : 0;
+    if (nzero)
+        *nzero = a.nzero;
     if (sh.tc == 8) {
-        a.ntiles = (a.f.cols + 7) / 8;
         return sh.nt == 512 ? launch_render_cfg<8, 512, 512, HBITS, HP2, CP2, 4, MODE>(a, s)
                             : launch_render_cfg<8, 1024, kChunk, HBITS, HP2, CP2, 4, MODE>(a, s);
     }
-    a.ntiles = (a.f.cols + 3) / 4;
     return sh.nt == 512 ? launch_render_cfg<4, 512, kChunk, HBITS, HP2, CP2, 4, MODE>(a, s)
                         : launch_render_cfg<4, 1024, kChunk, HBITS, HP2, CP2, 4, MODE>(a, s);
 }
@@ -1933,9 +1957,11 @@ int fsk_batch_run(const fsk_batch *b, hipStream_t s)
     return b->pow2 ? launch_batch<16, true, true>(b, s) : launch_batch<16, false, false>(b, s);
 }
 
-int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
-                      hipStream_t s)
+int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
+                            float *d_zero, size_t zero_n, int32_t *cleared, hipStream_t s)
 {
+    if (cleared)
+        *cleared = 0;
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
     if (f->rows > kMaxRows)
@@ -1950,12 +1976,32 @@ int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int6
     a.pitch = pitch;
     a.ntiles = 0;
     a.chunk_max = kChunk;
+    a.zero = nullptr;
+    a.zero_n4 = 0;
+    a.nzero = 0;
+    /* float4 stores only */
+    const bool z = d_zero && cleared && (((uintptr_t)d_zero) & 15u) == 0 && zero_n % 4 == 0;
+    a.zero = z ? (float4 *)d_zero : nullptr;
+    a.zero_n4 = z ? zero_n / 4 : 0;
+    a.nzero = 0;
+    int32_t nz = 0;
     bool hp2 = pow2(m->hrows) && pow2(m->hcols);
+    hipError_t e;
     if (m->hbits == 8)
-        return hp2 ? launch_render<8, true, false, kOutSlice>(a, s)
-                   : launch_render<8, false, false, kOutSlice>(a, s);
-    return hp2 ? launch_render<16, true, false, kOutSlice>(a, s)
-               : launch_render<16, false, false, kOutSlice>(a, s);
+        e = hp2 ? launch_render<8, true, false, kOutSlice>(a, s, &nz)
+                : launch_render<8, false, false, kOutSlice>(a, s, &nz);
+    else
+        e = hp2 ? launch_render<16, true, false, kOutSlice>(a, s, &nz)
+                : launch_render<16, false, false, kOutSlice>(a, s, &nz);
+    if (e == hipSuccess && cleared)
+        *cleared = nz > 0;
+    return e;
+}
+
+int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
+                      hipStream_t s)
+{
+    return fsk_render_slices_clear(m, f, d_src, pitch, nullptr, 0, nullptr, s);
 }
 
 static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, int32_t hc,
@@ -2000,6 +2046,9 @@ int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, in
     a.pitch = pitch;
     a.ntiles = 0;
     a.chunk_max = kChunk;
+    a.zero = nullptr;
+    a.zero_n4 = 0;
+    a.nzero = 0;
     return pow2(m->hrows) && pow2(m->hcols) ? launch_render<8, true, false, kOutNearest>(a, s)
                                             : launch_render<8, false, false, kOutNearest>(a, s);
 }
@@ -2242,6 +2291,9 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pi
     a.pitch = pitch;
     a.ntiles = 0;
     a.chunk_max = kChunk;
+    a.zero = nullptr;
+    a.zero_n4 = 0;
+    a.nzero = 0;
     bool hp2 = pow2(m->hrows) && pow2(m->hcols);
     bool cp2 = pow2(m->crows) && pow2(m->ccols);
     hipError_t e;

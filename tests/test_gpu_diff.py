@@ -314,3 +314,30 @@ def test_forward_tall_frame(ctx):
     cols = np.arange(v.w, dtype=np.int32)[None, :]
     assert np.array_equal(np.where(got_src < 0, -1, got_src * v.w + cols), want_src)
     assert np.array_equal(img.cpu().numpy(), want_img)
+
+
+@pytest.mark.parametrize("w", [128, 1600])
+def test_fused_step_gradient_clear(ctx, w):
+    """The fused step clears stale gradient planes either in spare
+    workgroups of the slice render (a narrow view leaves most CUs idle) or
+    inside the colour launch (a 1600-column view leaves too few): both
+    give the separate calls' result."""
+    r, col, _, _ = _setup(ctx)
+    v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=0.0, distance=300.0, fov=1.2,
+             h=40, w=w, d=0.01, z0=0.5, scale=64.0)
+    tp = torch.from_numpy(planes_of(col)).cuda()
+    target = torch.from_numpy(np.random.default_rng(w).random((3, v.h, v.w),
+                                                              dtype=np.float32)).cuda()
+    img, src = r.forward_raw(tp, 0.3, v)
+    gimg, loss = r.l2_loss_grad(img, target)
+    gp, gs = r.vjp_raw(tp, src, gimg, v)
+    assert float(gp.abs().sum()) > 0
+    for _ in range(2):
+        f_gp = torch.full_like(tp, 9.0)
+        out = r.l2_step(tp, 0.3, v, target, gplanes=f_gp, gsky=torch.full((1,), 4.0, device="cuda"),
+                        loss=torch.full((1,), 2.0, device="cuda"))
+        torch.cuda.synchronize()
+        assert torch.equal(out[0], img) and torch.equal(out[1], src)
+        assert _rel_err(f_gp.cpu().numpy(), gp.cpu().numpy()) <= 1e-5
+        assert abs(float(out[3][0]) - float(gs[0])) <= 1e-5 * max(abs(float(gs[0])), 1e-3)
+        assert abs(float(out[4][0]) - float(loss[0])) <= 1e-5 * float(loss[0])
