@@ -1880,8 +1880,7 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
     }
     bt->b.tune = ctx->tune;
     int32_t tc = fsk_tile_columns(max_rows, &bt->b.tune);
-    int64_t n_items = 0, n_slices = 0;
-    int32_t max_n = 0;
+    int64_t n_items = 0;
     for (int32_t i = 0; i < n_views; i++) {
         const struct fsx_map *m = maps[views[i].map];
         hv[i].m.hq = m->t->d_hq;
@@ -1896,29 +1895,15 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
         hv[i].offset = offsets[i];
         hv[i].ntiles = (views[i].w + tc - 1) / tc;
         n_items += hv[i].ntiles;
-        hv[i].tab = (int32_t)n_slices;
-        n_slices += hv[i].f.n > 0 ? hv[i].f.n : 0;
-        if (hv[i].f.n > max_n)
-            max_n = hv[i].f.n;
-    }
-    if (n_slices > INT32_MAX) {
-        free(bt);
-        free(hv);
-        set_error(ctx, "fsx_batch_new: more than 2^31 depth slices in one batch");
-        return 1;
     }
     size_t vb = sizeof(fsk_batch_view) * (size_t)n_views;
     size_t ib = sizeof(uint32_t) * (size_t)n_items;
     size_t mb = sizeof(uint32_t) * 2 * FS_QUEUES;
     size_t cb = sizeof(uint32_t) * FS_QUEUES;
-    /* slice tables: float4 records then inv_z, 16-byte aligned */
-    size_t tl = sizeof(float) * 4 * (size_t)(n_slices > 0 ? n_slices : 1);
-    size_t tz = sizeof(float) * (size_t)(n_slices > 0 ? n_slices : 1);
-    size_t head = (vb + ib + mb + cb + 255) & ~(size_t)255;
     uint32_t *hitems = (uint32_t *)malloc(ib + mb);
     int rc = 1;
     if (hitems && hipSetDevice(ctx->device) == hipSuccess &&
-        hipMalloc(&bt->d_mem, head + tl + tz) == hipSuccess) {
+        hipMalloc(&bt->d_mem, vb + ib + mb + cb) == hipSuccess) {
         deal_items(hv, n_views, hitems, hitems + n_items);
         char *base = (char *)bt->d_mem;
         hipError_t e = hipMemcpy(base, hv, vb, hipMemcpyHostToDevice);
@@ -1929,21 +1914,13 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
             bt->b.items = (const uint32_t *)(base + vb);
             bt->b.qmeta = (const uint32_t *)(base + vb + ib);
             bt->b.counters = (uint32_t *)(base + vb + ib + mb);
-            bt->b.tab_lines = base + head;
-            bt->b.tab_iz = (float *)(base + head + tl);
             bt->b.out = dev_out;
             bt->b.nqueues = FS_QUEUES;
             bt->b.max_rows = max_rows;
             bt->b.hbits = hbits;
             bt->b.pow2 = pow2;
             bt->b.grid = fsk_batch_grid(max_rows, &bt->b.tune);
-            e = (hipError_t)fsk_batch_tables(&bt->b, n_views, max_n, ctx->stream);
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(ctx->stream);
-            if (e == hipSuccess)
-                rc = 0;
-            else
-                set_error(ctx, "fsx_batch_new: slice tables: %s", hipGetErrorString(e));
+            rc = 0;
         } else {
             set_error(ctx, "fsx_batch_new: upload failed: %s", hipGetErrorString(e));
         }

@@ -71,7 +71,6 @@ typedef struct {
     fsk_frame f;
     int64_t offset;      /* first output element of the view */
     int32_t ntiles;
-    int32_t tab;         /* first slice record of the view in the batch tables */
 } fsk_batch_view;
 
 /* A prepared batch: the device arrays stay valid while it is launched. */
@@ -81,8 +80,6 @@ typedef struct {
     const uint32_t *qmeta;         /* device: qstart[nqueues], qlen[nqueues] */
     uint32_t *counters;            /* device: nqueues claim counters */
     uint32_t *out;
-    void *tab_lines;               /* device: float4 slice records (start, step) */
-    float *tab_iz;                 /* device: inv_z per slice record */
     int32_t nqueues, max_rows, hbits, pow2;
     int32_t grid;                  /* persistent workgroups */
     fsk_tuning tune;
@@ -94,8 +91,6 @@ int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t);
 int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t);
 /* Reset the claim counters and run the persistent batched render. */
 int fsk_batch_run(const fsk_batch *b, hipStream_t s);
-/* Fill the batch's slice tables (once, when the batch is prepared). */
-int fsk_batch_tables(const fsk_batch *b, int32_t n_views, int32_t max_n, hipStream_t s);
 /* Select the plain render's tuning (process-wide). */
 void fsk_set_tuning(const fsk_tuning *t);
 

@@ -488,9 +488,7 @@ template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int MODE = kO
           int GATHER = kGather, int CBATCH = kColourBatch>
 __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f, uint32_t *out,
                                             int64_t pitch, int32_t tile, int32_t chunk_max,
-                                            uint32_t *lds, TileShared<TC> &sh,
-                                            const float4 *gtl = nullptr,
-                                            const float *gtz = nullptr)
+                                            uint32_t *lds, TileShared<TC> &sh)
 {
     constexpr int NW = NT / 64;                  /* waves */
     constexpr int SG = NT / TC;                  /* slice groups */
@@ -534,19 +532,14 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         const int32_t cn = min(csz, n - base);
         /* phase 0: the chunk's slice records in slot order (slot j*SG + g
          * holds slice g*kr + j; slots past the chunk repeat its last
-         * slice), computed here or copied from the view's precomputed
-         * table (gtl/gtz, same values) */
+         * slice).  Computing them per tile is cheaper than copying a
+         * precomputed per-view table, even in batches. */
         const int32_t nslots = ((kr + GATHER - 1) / GATHER) * GATHER * SG;
         for (int32_t o = tid; o < nslots; o += NT) {
             const int32_t s = min((o % SG) * kr + o / SG, cn - 1);
-            if (gtl) {
-                tl[o] = gtl[base + s];
-                tz[o] = gtz[base + s];
-            } else {
-                Slice sl = slice_params(f, base + s);
-                tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
-                tz[o] = sl.iz;
-            }
+            Slice sl = slice_params(f, base + s);
+            tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
+            tz[o] = sl.iz;
         }
         if (tid == 0)
             sh.qn = 0u;
@@ -1242,24 +1235,8 @@ struct BatchArgs {
     const uint32_t *qmeta;
     uint32_t *counters;
     uint32_t *out;
-    const float4 *tab_lines;
-    const float *tab_iz;
     int32_t nqueues, chunk_max;
 };
-
-/* Slice records of every view of a batch, computed once when the batch is
- * prepared: view v's slice k is entry views[v].tab + k. */
-__global__ void k_batch_tables(const fsk_batch_view *__restrict__ views,
-                               float4 *__restrict__ lines, float *__restrict__ iz)
-{
-    const fsk_batch_view &bv = views[blockIdx.y];
-    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < bv.f.n;
-         k += gridDim.x * blockDim.x) {
-        Slice sl = slice_params(bv.f, k);
-        lines[bv.tab + k] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
-        iz[bv.tab + k] = sl.iz;
-    }
-}
 
 /* XCD this wave runs on (hardware register XCC_ID). */
 __device__ __forceinline__ uint32_t xcc_id()
@@ -1308,12 +1285,8 @@ __global__ __launch_bounds__(NT, WPE) void k_render_batch(BatchArgs b)
         if (it == 0xFFFFFFFFu)
             break;
         const fsk_batch_view &bv = b.views[it >> 16];
-        /* shallower gather batches than the single-view kernel: the
-         * persistent loop's own state would otherwise spill registers */
-        render_tile<TC, NT, CMAX, HBITS, HP2, CP2, kOutColour, (CMAX / (NT / TC) < 4 ? 2 : 4),
-                    1>(
-            bv.m, bv.f, b.out + bv.offset, bv.f.cols, it & 0xFFFFu, b.chunk_max, lds, sh,
-            b.tab_lines + bv.tab, b.tab_iz + bv.tab);
+        render_tile<TC, NT, CMAX, HBITS, HP2, CP2, kOutColour>(
+            bv.m, bv.f, b.out + bv.offset, bv.f.cols, it & 0xFFFFu, b.chunk_max, lds, sh);
     }
 }
 
@@ -1342,8 +1315,6 @@ hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
     b.qmeta = fb->qmeta;
     b.counters = fb->counters;
     b.out = fb->out;
-    b.tab_lines = (const float4 *)fb->tab_lines;
-    b.tab_iz = fb->tab_iz;
     b.nqueues = fb->nqueues;
     b.chunk_max = chunk;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, fb->counters, fb->nqueues);
@@ -1934,18 +1905,6 @@ int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t)
 void fsk_set_tuning(const fsk_tuning *t)
 {
     g_tuning = *t;
-}
-
-int fsk_batch_tables(const fsk_batch *b, int32_t n_views, int32_t max_n, hipStream_t s)
-{
-    if (n_views <= 0 || max_n <= 0)
-        return hipSuccess;
-    int32_t bx = (max_n + 255) / 256;
-    if (bx > 64)
-        bx = 64;
-    hipLaunchKernelGGL(k_batch_tables, dim3(bx, n_views), dim3(256), 0, s, b->views,
-                       (float4 *)b->tab_lines, (float *)b->tab_iz);
-    return hipGetLastError();
 }
 
 int fsk_batch_run(const fsk_batch *b, hipStream_t s)
