@@ -247,11 +247,13 @@ int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *
  * results as fsx_render_f32, then fsx_l2_loss against dev_target[3][h][w]
  * (dev_gimg = 2 (img - target), *dev_loss), then fsx_render_f32_vjp with
  * that gradient.  accumulate = 0 overwrites *dev_loss, dev_gplanes and
- * *dev_gsky, else adds to them.  The gradient clear shares a launch with
- * the forward colour pass and the loss is summed per workgroup in a fixed
- * order, so the loss of a step is reproducible; the gradients carry the
- * f32-atomic summation order of fsx_render_f32_vjp.  Asynchronous on the
- * context stream. */
+ * *dev_gsky, else adds to them.  Three launches: the visibility render
+ * (which also clears the gradient on CUs its tiles leave idle, else the
+ * colour pass does), the colour and loss pass, and the adjoint (which
+ * first sums the loss and sky partials).  The loss is summed per workgroup
+ * in a fixed order, so the loss of a step is reproducible; the gradients
+ * carry the f32-atomic summation order of fsx_render_f32_vjp.
+ * Asynchronous on the context stream. */
 int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
                            const struct fsx_view *view, const float *dev_planes,
                            const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
