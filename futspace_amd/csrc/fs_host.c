@@ -1841,8 +1841,12 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
                   const struct fsx_view *views, int32_t n_views, uint32_t *dev_out,
                   const int64_t *offsets, struct fsx_batch **out)
 {
-    if (!ctx || !maps || !views || !dev_out || !offsets || !out || n_views < 1 || n_maps < 1)
+    if (!ctx)
         return 1;
+    if (!maps || !views || !dev_out || !offsets || !out || n_views < 1 || n_maps < 1) {
+        set_error(ctx, "fsx_batch_new: null argument, or no views or maps");
+        return 1;
+    }
     if (n_views > 65535) {
         set_error(ctx, "fsx_batch_new: at most 65535 views per batch");
         return 1;

@@ -1,0 +1,89 @@
+"""Empty and degenerate inputs through the C-ABI (needs a GPU): frames
+with no rows or no columns render nothing and touch no memory, the
+state machine returns [0][w] / [h][0] frames, the differentiable entries
+overwrite (or keep, when accumulating) their outputs, and batches refuse
+empty views and empty view lists with a message."""
+import numpy as np
+import pytest
+
+from futspace_amd import FutspaceError, Session, View, scene
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from futspace_amd.diff import DiffRenderer, planes_of  # noqa: E402
+
+SHAPES = [(0, 16), (16, 0), (0, 0)]
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    col, hts = scene.scene_maps(5, 128)
+    return ctx.map(col, hts, shade_size=128), col, hts
+
+
+def _view(h, w):
+    return View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=60.0, distance=300.0,
+                fov=1.2, h=h, w=w, d=0.01, z0=0.5)
+
+
+@pytest.mark.parametrize("h,w", SHAPES)
+def test_empty_frames_write_nothing(ctx, small, h, w):
+    m = small[0]
+    v = _view(h, w)
+    assert ctx.render(m, v).shape == (h, w)
+    base = torch.full((4, 32), 7, dtype=torch.int32, device="cuda")
+    out = base[:h, :w] if h == 0 else base[:, :0]
+    v = _view(out.shape[0], out.shape[1])
+    ctx.render(m, v, out)
+    ctx.render2(m, v, out)
+    ctx.sync()
+    assert bool((base == 7).all())
+
+
+@pytest.mark.parametrize("h,w", SHAPES)
+def test_state_machine_empty_frames(ctx, small, h, w):
+    _, col, hts = small
+    s = Session(ctx)
+    try:
+        s.update_map(col, hts)
+        s.step()
+        s.resize(h, w)
+        assert s.render().shape == (h, w)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("accumulate", [False, True])
+@pytest.mark.parametrize("h,w", SHAPES)
+def test_differentiable_entries_on_empty_views(ctx, small, h, w, accumulate):
+    m, col, _ = small
+    r = DiffRenderer(ctx, m)
+    tp = torch.from_numpy(planes_of(col)).cuda()
+    v = _view(h, w)
+    img, src = r.forward_raw(tp, 0.3, v)
+    assert tuple(img.shape) == (3, h, w) and tuple(src.shape) == (h, w)
+    gp = torch.full_like(tp, 9.0)
+    gs = torch.full((1,), 4.0, device="cuda")
+    r.vjp_raw(tp, src, torch.empty((3, h, w), device="cuda"), v, gp, gs, accumulate=accumulate)
+    torch.cuda.synchronize()
+    fill = 9.0 if accumulate else 0.0
+    assert bool((gp == fill).all()) and float(gs[0]) == (4.0 if accumulate else 0.0)
+    gp.fill_(9.0)
+    gs.fill_(4.0)
+    loss = torch.full((1,), 2.0, device="cuda")
+    r.l2_step(tp, 0.3, v, torch.empty((3, h, w), device="cuda"), gplanes=gp, gsky=gs,
+              loss=loss, accumulate=accumulate)
+    torch.cuda.synchronize()
+    assert bool((gp == fill).all())
+    assert float(gs[0]) == (4.0 if accumulate else 0.0)
+    assert float(loss[0]) == (2.0 if accumulate else 0.0)
+
+
+def test_batches_refuse_empty_views(ctx, small):
+    m = small[0]
+    out = torch.zeros(64 * 64, dtype=torch.int32, device="cuda")
+    with pytest.raises(FutspaceError, match="unsupported size"):
+        ctx.render_batch([m], [_view(64, 64), _view(0, 64)], out, [0, 0])
+    with pytest.raises(FutspaceError, match="no views"):
+        ctx.render_batch([m], [], out, [])
