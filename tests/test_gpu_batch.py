@@ -149,3 +149,19 @@ def test_batch_with_a_tall_view(ctx, pow2_maps):
     shaded, alt = O.update_map(col, hts)
     v, o = views[-1], offs[-1]
     assert np.array_equal(flat[o:o + v.h * v.w].reshape(v.h, v.w), O.render_off(v, shaded, alt))
+
+
+def test_batch_sixteen_bit_heights(ctx):
+    """Maps with 16-bit heights (power-of-two and not) in one batch: every
+    view equals its single-view render."""
+    maps, shapes = [], []
+    for seed, size in ((21, 256), (22, 96)):
+        hts = scene.heightmap(seed, 256, 0.6, 900)[:size, :size].copy()
+        col = scene.colormap(seed, hts, 900)
+        maps.append(ctx.map(col, hts, height_bits=16))
+        shapes.append(hts.shape)
+    views = _views(20, shapes, seed=23, max_h=200)
+    for v in views:
+        v.height += 600.0
+    flat, offs = _run_batch(ctx, maps, views)
+    _check_views(ctx, maps, views, flat, offs)
