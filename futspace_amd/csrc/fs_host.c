@@ -87,6 +87,18 @@ static void set_error(struct futhark_context *ctx, const char *fmt, ...)
     ctx->error = strdup(buf);
 }
 
+/* A NULL context fails silently (there is nowhere to put a message);
+ * any other missing argument leaves one. */
+#define REQUIRE_ARGS(ctx, ok, fn)                                                   \
+    do {                                                                            \
+        if (!(ctx))                                                                 \
+            return 1;                                                               \
+        if (!(ok)) {                                                                \
+            set_error((ctx), "%s: null argument", (fn));                            \
+            return 1;                                                               \
+        }                                                                           \
+    } while (0)
+
 #define HIPCHECK(ctx, call)                                                         \
     do {                                                                            \
         hipError_t e_ = (hipError_t)(call);                                         \
@@ -366,8 +378,7 @@ const char *fsx_version(void)
 int fsx_selftest(struct futhark_context *ctx, int which, int64_t *mismatches,
                  int64_t *first_bad)
 {
-    if (!ctx || !mismatches)
-        return 1;
+    REQUIRE_ARGS(ctx, mismatches, "fsx_selftest");
     unsigned long long *d = NULL, h[2] = {0, 0};
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     if (dev_alloc(ctx, (void **)&d, sizeof h))
@@ -413,8 +424,7 @@ int fsx_profile_enable(struct futhark_context *ctx, int enable)
 
 int fsx_profile_read(struct futhark_context *ctx, double *total_ms, int64_t *launches)
 {
-    if (!ctx || !total_ms || !launches)
-        return 1;
+    REQUIRE_ARGS(ctx, total_ms && launches, "fsx_profile_read");
     double sum = 0.0;
     if (ctx->prof_n > 0) {
         HIPCHECK(ctx, hipEventSynchronize(ctx->prof_ev[2 * (ctx->prof_n - 1) + 1]));
@@ -512,8 +522,7 @@ int futhark_free_u32_2d(struct futhark_context *ctx, struct futhark_u32_2d *arr)
 
 int futhark_values_u32_2d(struct futhark_context *ctx, struct futhark_u32_2d *arr, uint32_t *data)
 {
-    if (!ctx || !arr)
-        return 1;
+    REQUIRE_ARGS(ctx, arr, "futhark_values_u32_2d");
     size_t bytes = (size_t)(arr->shape[0] * arr->shape[1]) * sizeof(uint32_t);
     if (bytes) {
         HIPCHECK(ctx, hipMemcpyAsync(data, arr->d, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -570,8 +579,7 @@ int futhark_free_i32_2d(struct futhark_context *ctx, struct futhark_i32_2d *arr)
 
 int futhark_values_i32_2d(struct futhark_context *ctx, struct futhark_i32_2d *arr, int32_t *data)
 {
-    if (!ctx || !arr)
-        return 1;
+    REQUIRE_ARGS(ctx, arr, "futhark_values_i32_2d");
     size_t bytes = (size_t)(arr->shape[0] * arr->shape[1]) * sizeof(int32_t);
     if (bytes) {
         HIPCHECK(ctx, hipMemcpyAsync(data, arr->d, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -895,8 +903,7 @@ int futhark_entry_init(struct futhark_context *ctx, struct futhark_opaque_state 
                        const uint32_t in0)
 {
     (void)in0;   /* the seed is unused by the reference */
-    if (!ctx || !out0)
-        return 1;
+    REQUIRE_ARGS(ctx, out0, "futhark_entry_init");
     struct futhark_opaque_state *s = (struct futhark_opaque_state *)calloc(1, sizeof *s);
     if (!s) {
         set_error(ctx, "init: out of host memory");
@@ -933,8 +940,7 @@ int futhark_entry_resize(struct futhark_context *ctx, struct futhark_opaque_stat
                          const int32_t in0, const int32_t in1,
                          const struct futhark_opaque_state *in2)
 {
-    if (!ctx || !out0 || !in2)
-        return 1;
+    REQUIRE_ARGS(ctx, out0 && in2, "futhark_entry_resize");
     struct futhark_opaque_state *o = state_copy(in2);
     if (!o)
         return 1;
@@ -967,8 +973,7 @@ int futhark_entry_key(struct futhark_context *ctx, struct futhark_opaque_state *
                       const int32_t in0, const int32_t in1,
                       const struct futhark_opaque_state *in2)
 {
-    if (!ctx || !out0 || !in2)
-        return 1;
+    REQUIRE_ARGS(ctx, out0 && in2, "futhark_entry_key");
     struct futhark_opaque_state *o = state_copy(in2);
     if (!o)
         return 1;
@@ -986,8 +991,7 @@ int futhark_entry_mouse(struct futhark_context *ctx, struct futhark_opaque_state
                         const struct futhark_opaque_state *in3)
 {
     (void)in0; (void)in1; (void)in2;
-    if (!ctx || !out0 || !in3)
-        return 1;
+    REQUIRE_ARGS(ctx, out0 && in3, "futhark_entry_mouse");
     *out0 = state_copy(in3);
     return *out0 ? 0 : 1;
 }
@@ -997,8 +1001,7 @@ int futhark_entry_wheel(struct futhark_context *ctx, struct futhark_opaque_state
                         const struct futhark_opaque_state *in2)
 {
     (void)in0; (void)in1;
-    if (!ctx || !out0 || !in2)
-        return 1;
+    REQUIRE_ARGS(ctx, out0 && in2, "futhark_entry_wheel");
     *out0 = state_copy(in2);
     return *out0 ? 0 : 1;
 }
@@ -1104,8 +1107,7 @@ int futhark_entry_step(struct futhark_context *ctx, struct futhark_opaque_state 
                        const float in0, const struct futhark_opaque_state *in1)
 {
     (void)in0;   /* the frame time is unused by the reference */
-    if (!ctx || !out0 || !in1)
-        return 1;
+    REQUIRE_ARGS(ctx, out0 && in1, "futhark_entry_step");
     struct futhark_opaque_state s = *in1;
     s.random = s.random + 0.005f;
     s.sky_color = argb_scale(0xFF9090E0u, s.sun_height);
@@ -1175,8 +1177,7 @@ int futhark_entry_text_content(struct futhark_context *ctx, float *out0, float *
                                float *out6, float *out7, float *out8,
                                const struct futhark_opaque_state *in0)
 {
-    if (!ctx || !in0)
-        return 1;
+    REQUIRE_ARGS(ctx, in0, "futhark_entry_text_content");
     *out0 = in0->cam_x;
     *out1 = in0->cam_y;
     *out2 = in0->cam_angle;
@@ -1195,8 +1196,7 @@ int futhark_entry_update_map(struct futhark_context *ctx, struct futhark_opaque_
                              const struct futhark_i32_2d *in1,
                              const struct futhark_opaque_state *in2)
 {
-    if (!ctx || !out0 || !in0 || !in1 || !in2)
-        return 1;
+    REQUIRE_ARGS(ctx, out0 && in0 && in1 && in2, "futhark_entry_update_map");
     if (in0->shape[0] != in1->shape[0] || in0->shape[1] != in1->shape[1]) {
         set_error(ctx, "update_map: colour map [%lld][%lld] and height map [%lld][%lld] differ",
                   (long long)in0->shape[0], (long long)in0->shape[1],
@@ -1248,8 +1248,7 @@ int futhark_entry_update_map(struct futhark_context *ctx, struct futhark_opaque_
 int futhark_entry_render(struct futhark_context *ctx, struct futhark_u32_2d **out0,
                          const struct futhark_opaque_state *in0)
 {
-    if (!ctx || !out0 || !in0)
-        return 1;
+    REQUIRE_ARGS(ctx, out0 && in0, "futhark_entry_render");
     if (in0->height < 0 || in0->width < 0) {
         set_error(ctx, "render: negative frame size");
         return 1;
@@ -1324,8 +1323,7 @@ int fsx_map_new(struct futhark_context *ctx, struct fsx_map **out,
                 const uint32_t *color, const int32_t *height, int64_t rows, int64_t cols,
                 int32_t height_bits, int32_t shade_size, float sun_height, float sun_ang)
 {
-    if (!ctx || !out || !color || !height)
-        return 1;
+    REQUIRE_ARGS(ctx, out && color && height, "fsx_map_new");
     if (rows <= 0 || cols <= 0 || rows > 65536 || cols > 65536) {
         set_error(ctx, "fsx_map_new: invalid map size %lld x %lld", (long long)rows, (long long)cols);
         return 1;
@@ -1386,8 +1384,7 @@ int32_t fsx_map_shade_size(const struct fsx_map *map)
 
 int fsx_map_shaded(struct futhark_context *ctx, const struct fsx_map *map, uint32_t *host_out)
 {
-    if (!ctx || !map || !host_out)
-        return 1;
+    REQUIRE_ARGS(ctx, map && host_out, "fsx_map_shaded");
     size_t n = (size_t)map->s->rows * (size_t)map->s->cols;
     HIPCHECK(ctx, hipMemcpyAsync(host_out, map->s->d_shaded, n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1397,8 +1394,7 @@ int fsx_map_shaded(struct futhark_context *ctx, const struct fsx_map *map, uint3
 int fsx_render(struct futhark_context *ctx, const struct fsx_map *map,
                const struct fsx_view *view, uint32_t *dev_out, int64_t pitch)
 {
-    if (!ctx || !map || !view || (!dev_out && view->h > 0 && view->w > 0))
-        return 1;
+    REQUIRE_ARGS(ctx, map && view && (dev_out || view->h <= 0 || view->w <= 0), "fsx_render");
     if (pitch != 0 && pitch < view->w) {
         set_error(ctx, "fsx_render: pitch %lld below width %d", (long long)pitch, view->w);
         return 1;
@@ -1414,8 +1410,7 @@ int fsx_render(struct futhark_context *ctx, const struct fsx_map *map,
 int fsx_render2(struct futhark_context *ctx, const struct fsx_map *map,
                 const struct fsx_view *view, uint32_t *dev_out, int64_t pitch)
 {
-    if (!ctx || !map || !view)
-        return 1;
+    REQUIRE_ARGS(ctx, map && view, "fsx_render2");
     if (view->h < 0 || view->w < 0 || (pitch != 0 && pitch < view->w)) {
         set_error(ctx, "fsx_render2: bad frame %d x %d (pitch %lld)", view->h, view->w,
                   (long long)pitch);
@@ -1480,8 +1475,7 @@ int fsx_render2(struct futhark_context *ctx, const struct fsx_map *map,
 int fsx_render_host(struct futhark_context *ctx, const struct fsx_map *map,
                     const struct fsx_view *view, uint32_t *host_out)
 {
-    if (!ctx || !map || !view || !host_out)
-        return 1;
+    REQUIRE_ARGS(ctx, map && view && host_out, "fsx_render_host");
     size_t bytes = (size_t)(view->h > 0 ? view->h : 0) * (size_t)(view->w > 0 ? view->w : 0) * 4;
     if (bytes == 0)
         return render_into(ctx, map->t, map->s, view, NULL, 0);
@@ -1520,8 +1514,7 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
                          const int32_t *dev_height, int32_t q, int32_t r,
                          const struct fsx_view *view, int32_t *dev_out)
 {
-    if (!ctx || !view)
-        return 1;
+    REQUIRE_ARGS(ctx, view, "fsx_render_stateless");
     if (view->h < 0 || view->w < 0 || view->h > fsk_render_max_rows() || q <= 0 || r <= 0 ||
         (int64_t)q * r > ((int64_t)1 << 30)) {
         set_error(ctx, "fsx_render_stateless: unsupported sizes (frame %d x %d, map %d x %d)",
@@ -1561,8 +1554,7 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
 
 int fsx_map_shade_factor(struct futhark_context *ctx, const struct fsx_map *map, float *dev_sf)
 {
-    if (!ctx || !map || !dev_sf)
-        return 1;
+    REQUIRE_ARGS(ctx, map && dev_sf, "fsx_map_shade_factor");
     const struct fs_terrain *t = map->t;
     const struct fs_shade *s = map->s;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
@@ -1606,8 +1598,7 @@ int fsx_render_f32(struct futhark_context *ctx, const struct fsx_map *map,
                    const struct fsx_view *view, const float *dev_planes, const float *dev_sf,
                    int32_t hc, int32_t wc, float sky_scale, float *dev_img, int32_t *dev_src)
 {
-    if (!ctx || !map || !view)
-        return 1;
+    REQUIRE_ARGS(ctx, map && view, "fsx_render_f32");
     fsk_frame f;
     if (diff_frame(ctx, "fsx_render_f32", view, hc, wc, &f))
         return 1;
@@ -1639,8 +1630,7 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
                        const int32_t *dev_src, const float *dev_gimg, int32_t accumulate,
                        float *dev_gplanes, float *dev_gsky)
 {
-    if (!ctx || !view)
-        return 1;
+    REQUIRE_ARGS(ctx, view, "fsx_render_f32_vjp");
     fsk_frame f;
     if (diff_frame(ctx, "fsx_render_f32_vjp", view, hc, wc, &f))
         return 1;
@@ -1673,8 +1663,7 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
                            int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
                            float *dev_gsky, float *dev_loss)
 {
-    if (!ctx || !map || !view)
-        return 1;
+    REQUIRE_ARGS(ctx, map && view, "fsx_render_f32_l2_step");
     fsk_frame f;
     if (diff_frame(ctx, "fsx_render_f32_l2_step", view, hc, wc, &f))
         return 1;
@@ -1963,8 +1952,7 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
 
 int fsx_batch_run(struct futhark_context *ctx, struct fsx_batch *batch)
 {
-    if (!ctx || !batch)
-        return 1;
+    REQUIRE_ARGS(ctx, batch, "fsx_batch_run");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
     HIPCHECK(ctx, fsk_batch_run(&batch->b, ctx->stream));
@@ -1974,8 +1962,7 @@ int fsx_batch_run(struct futhark_context *ctx, struct fsx_batch *batch)
 
 int fsx_batch_capture(struct futhark_context *ctx, struct fsx_batch *batch)
 {
-    if (!ctx || !batch)
-        return 1;
+    REQUIRE_ARGS(ctx, batch, "fsx_batch_capture");
     if (ctx->stream == NULL) {
         set_error(ctx, "fsx_batch_capture: the context stream cannot be captured");
         return 1;

@@ -98,3 +98,33 @@ def test_product_never_imports_the_oracle():
                 text = open(os.path.join(dirpath, f)).read()
                 assert "fs_oracle" not in text and "import oracle" not in text, f
                 assert "from oracle" not in text, f
+
+
+def test_every_entry_refuses_a_null_context():
+    """Every int-returning entry of the header returns nonzero for a NULL
+    context (before touching HIP, so no GPU is needed); the array
+    constructors, the error getter and the stream getter return NULL."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    decls = re.findall(r"\n(int|[^\n;(]*\*)\s*((?:futhark|fsx)_\w+)\s*\(([^;]*?)\);", text)
+    lib = ctypes.CDLL(LIB)
+    checked = 0
+    for ret, name, args in decls:
+        nargs = 0 if args.strip() in ("", "void") else args.count(",") + 1
+        if nargs == 0 or not args.lstrip().startswith("struct futhark_context *"):
+            continue
+        fn = getattr(lib, name)
+        argv = [ctypes.c_void_p(None)] * nargs
+        if ret.strip() == "int":
+            fn.restype = ctypes.c_int
+            assert fn(*argv) != 0, name
+        else:
+            fn.restype = ctypes.c_void_p
+            assert fn(*argv) is None, name
+        checked += 1
+    assert checked >= 46
+    for name in ("futhark_context_free", "futhark_context_pause_profiling",
+                 "futhark_context_unpause_profiling", "futhark_debugging_report"):
+        getattr(lib, name).restype = None
+        getattr(lib, name)(ctypes.c_void_p(None))
+    lib.fsx_map_shade_size.restype = ctypes.c_int32
+    assert lib.fsx_map_shade_size(ctypes.c_void_p(None)) == 0

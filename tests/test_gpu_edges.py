@@ -87,3 +87,28 @@ def test_batches_refuse_empty_views(ctx, small):
         ctx.render_batch([m], [_view(64, 64), _view(0, 64)], out, [0, 0])
     with pytest.raises(FutspaceError, match="no views"):
         ctx.render_batch([m], [], out, [])
+
+
+def test_invalid_arguments_leave_a_message(ctx, small):
+    """Bad shapes, a pitch below the width and null arguments fail with a
+    message for futhark_context_get_error; the context stays usable."""
+    import ctypes
+    lib, c, m = ctx.lib, ctx.ptr, small[0]
+    assert not lib.futhark_new_u32_2d(c, None, -1, 4)
+    assert "invalid array shape" in ctx.error()
+    assert not lib.futhark_new_i32_2d(c, None, 1 << 20, 1 << 20)
+    assert "invalid array shape" in ctx.error()
+    out = torch.zeros((8, 16), dtype=torch.int32, device="cuda")
+    cv = _view(8, 16).to_c()
+    assert lib.fsx_render(c, m.ptr, ctypes.byref(cv), ctypes.c_void_p(out.data_ptr()), 8) != 0
+    assert "pitch" in ctx.error()
+    assert lib.fsx_render(c, m.ptr, ctypes.byref(cv), None, 0) != 0
+    assert "fsx_render: null argument" in ctx.error()
+    arr = ctypes.c_void_p()
+    assert lib.futhark_entry_render(c, ctypes.byref(arr), None) != 0
+    assert "futhark_entry_render: null argument" in ctx.error()
+    assert lib.fsx_batch_run(c, None) != 0
+    assert "null argument" in ctx.error()
+    ctx.render(m, _view(8, 16), out)
+    ctx.sync()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ctx.render(m, _view(8, 16)))
