@@ -54,8 +54,6 @@ int fsk_render2(const fsk_map *m, const fsk_frame *f, const float *d_cs, const f
  * own slice table per depth chunk. */
 int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
                hipStream_t s);
-/* LDS bytes the render kernel needs for a frame of `rows` rows. */
-size_t fsk_render_lds_bytes(int32_t rows);
 int32_t fsk_render_max_rows(void);
 
 /* Tuning knobs of the render kernels (0 = automatic). */

@@ -1884,13 +1884,6 @@ inline bool pow2(int32_t v)
 
 extern "C" {
 
-size_t fsk_render_lds_bytes(int32_t rows)
-{
-    Shape sh = resolve_shape(rows, current_tuning());
-    return sh.tc == 8 ? render_lds_bytes_tc<8>(rows, sh.chunk)
-                      : render_lds_bytes_tc<4>(rows, sh.chunk);
-}
-
 int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t)
 {
     return resolve_shape(max_rows, *t).tc;
