@@ -262,9 +262,10 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
                            float *dev_gsky, float *dev_loss);
 
 /* Render kernel shape, as chosen by the autotuner: columns per workgroup
- * tile (4 or 8), workgroups per CU for 4-column tiles (1 or 2) and depth
- * slices per chunk (256-2048); 0 leaves a knob automatic.  Applies to
- * later renders and to batches prepared afterwards. */
+ * tile (4 or 8), workgroups per CU (1 of 1024 threads, or 2 of 512 whose
+ * depth chunks are shortened until two tiles fit in LDS) and depth slices
+ * per chunk (256-2048); 0 leaves a knob automatic.  Applies to later
+ * renders and to batches prepared afterwards. */
 int fsx_set_tuning(struct futhark_context *ctx, int32_t tile_columns, int32_t wg_per_cu,
                    int32_t chunk);
 
