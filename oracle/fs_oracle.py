@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfsoracle.so")
+# FSO_LIB selects another build of the same source (`make -C oracle asan`)
+LIB_PATH = os.environ.get("FSO_LIB") or os.path.join(_HERE, "libfsoracle.so")
 
 
 class Camera(ctypes.Structure):
