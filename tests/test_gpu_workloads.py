@@ -65,3 +65,39 @@ def test_every_view_of_the_256_view_batch(ctx):
 
     bad = _differences(pairs())
     assert not bad, f"views that differ (index, pixels): {bad[:8]}"
+
+
+def test_fused_step_of_the_720p_workload(ctx):
+    """The differentiable benchmark's step at full size (1280x720 view of
+    the 2048^2 map, 2048 slices, f32 planes [3][2048][2048]): shade factor,
+    image and visible slices bit-exact, the loss within f32 summation
+    order, plane gradients within 1e-5 of the largest and the sky gradient
+    within 1e-5 of the mass of its terms."""
+    from futspace_amd.diff import DiffRenderer, planes_of
+    col, hts = scene.diff_2048_maps()
+    m = ctx.map(col, hts, shade_size=2048)
+    r = DiffRenderer(ctx, m)
+    v = scene.diff_2048_view(hts)
+    alt = np.asarray(hts, dtype=np.int32) & 255
+    sf = O.shade_factor(O.shadow_counts(alt, O.sun(0.1, 0.1), 2048))
+    torch.cuda.synchronize()
+    assert np.array_equal(r.sf.cpu().numpy(), sf)
+    planes = planes_of(col)
+    target, _ = O.render_f32(v, scene.perturbed_planes(planes, scene.DIFF_2048_TARGET_SEED), sf,
+                             alt, scene.SUN_HEIGHT)
+    img, src, gp, gs, loss = r.l2_step(torch.from_numpy(planes).cuda(), scene.SUN_HEIGHT, v,
+                                       torch.from_numpy(target).cuda())
+    torch.cuda.synchronize()
+    want_img, want_src = O.render_f32(v, planes, sf, alt, scene.SUN_HEIGHT)
+    assert np.array_equal(img.cpu().numpy(), want_img)
+    got_src = src.cpu().numpy()
+    cols = np.arange(v.w, dtype=np.int32)[None, :]
+    assert np.array_equal(np.where(got_src < 0, -1, got_src * v.w + cols), want_src)
+    d = want_img.astype(np.float64) - target
+    assert abs(float(loss[0]) - float((d * d).sum())) <= 1e-5 * float((d * d).sum())
+    g = 2.0 * (want_img - target)
+    want_gp, want_gs = O.render_f32_vjp(v, planes, sf, want_src, g)
+    got_gp = gp.cpu().numpy().astype(np.float64)
+    assert float(np.abs(got_gp - want_gp).max()) <= 1e-5 * float(np.abs(want_gp).max())
+    sky_terms = np.abs(g[:, want_src < 0]).sum(axis=1) @ np.array([144, 144, 224]) / 255.0
+    assert abs(float(gs[0]) - want_gs) <= 1e-5 * max(sky_terms, 1e-3)
