@@ -67,24 +67,21 @@ def test_every_view_of_the_256_view_batch(ctx):
     assert not bad, f"views that differ (index, pixels): {bad[:8]}"
 
 
-def test_fused_step_of_the_720p_workload(ctx):
-    """The differentiable benchmark's step at full size (1280x720 view of
-    the 2048^2 map, 2048 slices, f32 planes [3][2048][2048]): shade factor,
-    image and visible slices bit-exact, the loss within f32 summation
-    order, plane gradients within 1e-5 of the largest and the sky gradient
-    within 1e-5 of the mass of its terms."""
+def _check_fused_step(ctx, col, hts, bits, v, target_seed):
+    """Shade factor, image and visible slices bit-exact, the loss within f32
+    summation order, plane gradients within 1e-5 of the largest and the sky
+    gradient within 1e-5 of the mass of its terms."""
     from futspace_amd.diff import DiffRenderer, planes_of
-    col, hts = scene.diff_2048_maps()
-    m = ctx.map(col, hts, shade_size=2048)
+    size = hts.shape[0]
+    m = ctx.map(col, hts, height_bits=bits, shade_size=size)
     r = DiffRenderer(ctx, m)
-    v = scene.diff_2048_view(hts)
-    alt = np.asarray(hts, dtype=np.int32) & 255
-    sf = O.shade_factor(O.shadow_counts(alt, O.sun(0.1, 0.1), 2048))
+    alt = np.asarray(hts, dtype=np.int32) & ((1 << bits) - 1)
+    sf = O.shade_factor(O.shadow_counts(alt, O.sun(0.1, 0.1), size))
     torch.cuda.synchronize()
     assert np.array_equal(r.sf.cpu().numpy(), sf)
     planes = planes_of(col)
-    target, _ = O.render_f32(v, scene.perturbed_planes(planes, scene.DIFF_2048_TARGET_SEED), sf,
-                             alt, scene.SUN_HEIGHT)
+    target, _ = O.render_f32(v, scene.perturbed_planes(planes, target_seed), sf, alt,
+                             scene.SUN_HEIGHT)
     img, src, gp, gs, loss = r.l2_step(torch.from_numpy(planes).cuda(), scene.SUN_HEIGHT, v,
                                        torch.from_numpy(target).cuda())
     torch.cuda.synchronize()
@@ -101,3 +98,19 @@ def test_fused_step_of_the_720p_workload(ctx):
     assert float(np.abs(got_gp - want_gp).max()) <= 1e-5 * float(np.abs(want_gp).max())
     sky_terms = np.abs(g[:, want_src < 0]).sum(axis=1) @ np.array([144, 144, 224]) / 255.0
     assert abs(float(gs[0]) - want_gs) <= 1e-5 * max(sky_terms, 1e-3)
+
+
+def test_fused_step_of_the_720p_workload(ctx):
+    """The differentiable benchmark's step at full size: the 1280x720 view
+    of the 2048^2 map, 2048 slices, f32 planes [3][2048][2048]."""
+    col, hts = scene.diff_2048_maps()
+    _check_fused_step(ctx, col, hts, 8, scene.diff_2048_view(hts), scene.DIFF_2048_TARGET_SEED)
+
+
+def test_fused_step_of_a_4k_fitting_view(ctx):
+    """One view of the fitting workload at full size: 3840x2160, 4096
+    slices (several depth chunks), the 8192^2 map with 16-bit heights and
+    f32 planes [3][8192][8192]."""
+    col, hts = scene.diff_8192_maps()
+    v = scene.diff_8192_views(hts)[5]
+    _check_fused_step(ctx, col, hts, 16, v, scene.DIFF_8192_SEED)
