@@ -797,6 +797,19 @@ static int32_t zs_count(float c, float d, float z0)
     return n < 0 ? 0 : n;
 }
 
+/* Depth slices one render may march: 4096 times the largest workload
+ * measured here; a longer march would keep each tile busy for seconds. */
+#define FS_MAX_SLICES (1 << 24)
+
+static int slices_ok(struct futhark_context *ctx, const char *what, int32_t n)
+{
+    if (n > FS_MAX_SLICES) {
+        set_error(ctx, "%s: %d depth slices, above the supported %d", what, n, FS_MAX_SLICES);
+        return 0;
+    }
+    return 1;
+}
+
 static void frame_of_view(const struct fsx_view *v, fsk_frame *f)
 {
     float sin_ang = sinf(v->angle), cos_ang = cosf(v->angle), view = v->fov;
@@ -835,6 +848,8 @@ static int render_into(struct futhark_context *ctx, const struct fs_terrain *t,
         return 0;
     fsk_frame f;
     frame_of_view(v, &f);
+    if (!slices_ok(ctx, "render", f.n))
+        return 1;
     fsk_map m;
     m.hq = t->d_hq;
     m.hrows = t->rows;
@@ -1533,6 +1548,8 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
     f.z0 = 1.0f;
     f.scale = 240.0f;
     f.n = stateless_zs_count(view->distance);
+    if (!slices_ok(ctx, "fsx_render_stateless", f.n))
+        return 1;
     fsk_map m;
     m.hq = dev_height;
     m.hrows = q;
@@ -1591,7 +1608,7 @@ static int diff_frame(struct futhark_context *ctx, const char *what, const struc
         return 1;
     }
     frame_of_view(v, f);
-    return 0;
+    return slices_ok(ctx, what, f->n) ? 0 : 1;
 }
 
 int fsx_render_f32(struct futhark_context *ctx, const struct fsx_map *map,
@@ -1884,6 +1901,14 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
         hv[i].m.crows = m->s->rows;
         hv[i].m.ccols = m->s->cols;
         frame_of_view(&views[i], &hv[i].f);
+        if (hv[i].f.n > FS_MAX_SLICES) {
+            const int32_t n = hv[i].f.n;
+            free(bt);
+            free(hv);
+            set_error(ctx, "fsx_batch_new: view %d has %d depth slices, above the supported %d", i,
+                      n, FS_MAX_SLICES);
+            return 1;
+        }
         hv[i].f.tame = fsk_frame_tame(&hv[i].f);
         hv[i].offset = offsets[i];
         hv[i].ntiles = (views[i].w + tc - 1) / tc;

@@ -125,8 +125,9 @@ int futhark_entry_update_map(struct futhark_context *ctx, struct futhark_opaque_
  * the frame size, and the depth/projection parameters the reference
  * hard-codes -- d (invz_param1, fut/voxel_renderer.fut:104), z0
  * (invz_param2, :103), scale (magic_number, :217; <= 0 means f32(w/2)),
- * n_slices (<= 0 means get_zs' count from distance, :28-34).  `map`
- * indexes the map list of a batched render. */
+ * n_slices (<= 0 means get_zs' count from distance, :28-34; a render
+ * refuses more than 2^24 slices, and frames of more than 4096 rows).
+ * `map` indexes the map list of a batched render. */
 struct fsx_view {
     float x, y, height, angle, horizon, distance, fov;
     uint32_t sky_color;

@@ -112,3 +112,25 @@ def test_invalid_arguments_leave_a_message(ctx, small):
     ctx.render(m, _view(8, 16), out)
     ctx.sync()
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ctx.render(m, _view(8, 16)))
+
+
+def test_depth_slice_limit(ctx, small):
+    """More than 2^24 depth slices -- given outright or implied by a tiny
+    slice spacing -- is refused with a message by every render entry."""
+    m, col, _ = small
+    huge = _view(8, 16)
+    huge.n_slices = (1 << 24) + 1
+    implied = _view(8, 16)
+    implied.d, implied.z0 = 1e-12, 0.0      # get_zs: about 4.5e7 slices to distance 300
+    for v in (huge, implied):
+        with pytest.raises(FutspaceError, match="depth slices"):
+            ctx.render(m, v)
+        r = DiffRenderer(ctx, m)
+        with pytest.raises(FutspaceError, match="depth slices"):
+            r.forward_raw(torch.from_numpy(planes_of(col)).cuda(), 0.3, v)
+        out = torch.zeros(8 * 16, dtype=torch.int32, device="cuda")
+        with pytest.raises(FutspaceError, match="depth slices"):
+            ctx.render_batch([m], [v], out, [0])
+    ok = _view(8, 16)
+    ok.n_slices = 1 << 12
+    assert ctx.render(m, ok).shape == (8, 16)
