@@ -136,6 +136,40 @@ def bench_batched(ctx, reps):
     }
 
 
+def bench_host_frames(ctx, m, views, n):
+    """The PCIe-inclusive rate (not `value`): each fly-through frame
+    rendered and copied to host memory by fsx_render_host, as a frame loop
+    that reads every frame back (futhark_values) would do; into pinned and
+    into pageable host buffers."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from futspace_amd import _native
+
+    lib = _native.futspace()
+    H, W = views[0].h, views[0].w
+    pinned = torch.empty((H, W), dtype=torch.int32, pin_memory=True)
+    pageable = np.empty((H, W), dtype=np.uint32)
+
+    def rate(ptr):
+        def one(v):
+            cv = v.to_c()
+            ctx.check(lib.fsx_render_host(ctx.ptr, m.ptr, ctypes.byref(cv), ptr),
+                      "fsx_render_host")
+        for v in views[:3]:
+            one(v)
+        t0 = time.perf_counter()
+        for i in range(n):
+            one(views[i % len(views)])
+        return n / (time.perf_counter() - t0)
+
+    return {"pinned_frames_per_s": rate(ctypes.c_void_p(pinned.data_ptr())),
+            "pageable_frames_per_s": rate(ctypes.c_void_p(pageable.ctypes.data)),
+            "frames": n, "frame_bytes": H * W * 4}
+
+
 def _kernel_ms(ctx, fn, reps):
     ctx.sync()
     ctx.profile(True)
@@ -379,6 +413,7 @@ def main():
         ctx.set_stream(None)
         rec["batched"] = bench_batched(ctx, max(3, args.batch_reps if not args.quick else 3))
     ctx.set_stream(None)
+    rec["host_frames"] = bench_host_frames(ctx, m, views, 5 if args.quick else 60)
     if not args.no_diff:
         rec["diff"] = bench_diff(ctx, 50 if not args.quick else 5)
     if not args.no_fit:
