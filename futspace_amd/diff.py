@@ -36,6 +36,8 @@ class DiffRenderer:
         self.map = m
         n = m.shade_size
         self.hc = self.wc = n
+        self._depth = 0          # nesting of stream()
+        self._ext = None
         self.sf = torch.empty((n, n), dtype=torch.float32, device="cuda")
         with self.stream():
             ctx.check(ctx.lib.fsx_map_shade_factor(ctx.ptr, m.ptr, _ptr(self.sf)),
@@ -43,15 +45,28 @@ class DiffRenderer:
 
     @contextlib.contextmanager
     def stream(self):
+        """The context's HIP stream, ordered after torch's current stream on
+        entry and before it on exit.  Nested uses order the two streams only
+        at the outermost entry and exit: torch work issued inside must not
+        read this renderer's outputs before the outermost exit."""
+        if self._depth:
+            self._depth += 1
+            try:
+                yield self._ext
+            finally:
+                self._depth -= 1
+            return
         handle = self.ctx.lib.fsx_context_stream(self.ctx.ptr)
         if not handle:
             raise FutspaceError("the renderer context has no stream")
         ext = torch.cuda.ExternalStream(handle)
         cur = torch.cuda.current_stream()
         ext.wait_stream(cur)
+        self._ext, self._depth = ext, 1
         try:
             yield ext
         finally:
+            self._ext, self._depth = None, 0
             cur.wait_stream(ext)
 
     def _check_planes(self, planes):
@@ -193,13 +208,16 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
     img = gimg = src = None
     losses = torch.zeros(steps, dtype=torch.float32, device="cuda")
     for it in range(steps):
-        for i, (v, t) in enumerate(zip(views, targets)):
-            if img is None or tuple(img.shape) != (3, v.h, v.w):
-                img = torch.empty((3, v.h, v.w), dtype=torch.float32, device="cuda")
-                gimg = torch.empty_like(img)
-                src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
-            renderer.l2_step(planes, sky_scale, v, t, img, src, gimg, grad, gsky,
-                             losses[it:it + 1], accumulate=i > 0)
+        # one ordering of the renderer's stream against torch's per step:
+        # the views' calls queue back to back, Adam follows them
+        with renderer.stream():
+            for i, (v, t) in enumerate(zip(views, targets)):
+                if img is None or tuple(img.shape) != (3, v.h, v.w):
+                    img = torch.empty((3, v.h, v.w), dtype=torch.float32, device="cuda")
+                    gimg = torch.empty_like(img)
+                    src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
+                renderer.l2_step(planes, sky_scale, v, t, img, src, gimg, grad, gsky,
+                                 losses[it:it + 1], accumulate=i > 0)
         planes.grad = grad
         opt.step()
         if on_step is not None:
