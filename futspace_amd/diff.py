@@ -202,7 +202,11 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
     views accumulate into one buffer through the adjoint's scatter-add.
     Returns the per-step losses (a CUDA tensor, read once at the end)."""
     planes.requires_grad_(False)
-    opt = torch.optim.Adam([planes], lr=lr)
+    try:
+        # one pass over parameter, gradient and moments per step
+        opt = torch.optim.Adam([planes], lr=lr, fused=True)
+    except (RuntimeError, TypeError):
+        opt = torch.optim.Adam([planes], lr=lr)
     grad = torch.zeros_like(planes)
     gsky = torch.zeros(1, dtype=torch.float32, device="cuda")
     img = gimg = src = None
