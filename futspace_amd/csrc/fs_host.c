@@ -263,6 +263,12 @@ struct futhark_context *futhark_context_new(struct futhark_context_config *cfg)
         return NULL;
     }
     ctx->stream = ctx->own_stream;
+    if (fsk_device_init() != hipSuccess) {
+        fprintf(stderr, "futspace: device table upload failed\n");
+        hipStreamDestroy(ctx->own_stream);
+        free(ctx);
+        return NULL;
+    }
     ctx->empty_terrain = terrain_zero(ctx);
     ctx->empty_shade = shade_zero(ctx);
     if (!ctx->empty_terrain || !ctx->empty_shade) {

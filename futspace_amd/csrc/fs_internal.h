@@ -44,6 +44,11 @@ typedef struct {
  * projected row below 2^30 in magnitude for any 16-bit height. */
 int32_t fsk_frame_tame(const fsk_frame *f);
 
+/* Once per device, before its first render: uploads the kernels' constant
+ * tables (the byte quantisation table of the colour path).  Thread-safe;
+ * later calls on the same device return at once. */
+int fsk_device_init(void);
+
 /* render2 (ray march) of one frame into out[rows][pitch]: d_cs/d_sn[cols]
  * the host's cos / sin of angle + horz_ang per column, d_color the map's
  * unshaded colours [hrows][hcols], n = i32(distance / 4). */
@@ -92,7 +97,7 @@ int fsk_batch_run(const fsk_batch *b, hipStream_t s);
 /* Select the plain render's tuning (process-wide). */
 void fsk_set_tuning(const fsk_tuning *t);
 
-/* Device self-test `which` (0-3) of the arithmetic shortcuts; adds the
+/* Device self-test `which` (0-6) of the arithmetic shortcuts; adds the
  * number of mismatching inputs to *d_bad. */
 int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s);
 

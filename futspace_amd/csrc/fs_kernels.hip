@@ -21,7 +21,10 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "fs_internal.h"
@@ -224,6 +227,50 @@ __device__ __forceinline__ Slice slice_params(const fsk_frame &f, int32_t k)
     return s;
 }
 
+/* RN(1 / z) for z > 0 whose reciprocal is a normal number: the hardware
+ * reciprocal (within an ulp) and one Newton step with FMA.  Device self-test
+ * 5 compares it with the correctly rounded division for every such z. */
+__device__ __forceinline__ float rcp_rn(float z)
+{
+    float y = __builtin_amdgcn_rcpf(z);
+    float e = __builtin_fmaf(-z, y, 1.0f);
+    return __builtin_fmaf(e, y, y);
+}
+
+/* RN(a / b) from rb = RN(1 / b), for finite a and b >= 1 (Markstein's
+ * correction: q = RN(a * rb) is within an ulp of a / b, the residual
+ * a - q * b is exact in one FMA, and q + residual * rb rounds to the
+ * nearest quotient).  The residual is not exact for quotients near the
+ * subnormal range, and the signs of zero differ, so quotients below 2^-100
+ * in magnitude take the division itself.  Device self-test 6 compares it
+ * with the correctly rounded division for every finite f32 a at a set of
+ * frame widths b. */
+__device__ __forceinline__ float div_rn(float a, float b, float rb)
+{
+    float q = a * rb;
+    float r = __builtin_fmaf(-q, b, a);
+    q = __builtin_fmaf(r, rb, q);
+    return fabsf(q) >= 0x1p-100f ? q : a / b;
+}
+
+/* slice_params of a tame frame (z >= z0 > 0, finite scalars) with the two
+ * divisions by the frame width and the reciprocal of z taken by rcp_rn and
+ * div_rn; rfw = RN(1 / fw).  Bit-identical to slice_params there. */
+__device__ __forceinline__ Slice slice_params_tame(const fsk_frame &f, int32_t k, float rfw)
+{
+    float i = (float)(k + 1);
+    float z = (i / 2.0f) * ((2.0f * f.z0) + ((i - 1.0f) * f.d));
+    float left_x = f.Lx * z, left_y = f.Ly * z;
+    float right_x = f.Rx * z, right_y = f.Ry * z;
+    Slice s;
+    s.dx = div_rn(right_x - left_x, f.fw, rfw);
+    s.dy = div_rn(right_y - left_y, f.fw, rfw);
+    s.sx = left_x + f.cam_x;
+    s.sy = left_y + f.cam_y;
+    s.iz = rcp_rn(z) * f.scale;
+    return s;
+}
+
 /* ------------------------------------------------------------------ */
 /* The render kernel                                                   */
 /* ------------------------------------------------------------------ */
@@ -324,6 +371,50 @@ __device__ __forceinline__ uint32_t mix_channel_exact(float m1, float s1, float 
     return (uint32_t)(s * 255.0f);
 }
 
+/* The byte quantisation of a mixed channel, u32.f32(sqrt(t) * 255), by
+ * table: the map t -> byte is non-decreasing on [0, 1], so the float bits
+ * of t (clamped below at kQLo, where the byte is still 0) fall into one of
+ * kQBuckets buckets of 2^16 consecutive bit patterns, and no bucket holds
+ * more than one step of the byte.  Entry i of the table, with
+ * u = max(bits(t) - kQLo, 0) in bucket i = u >> 16, is
+ *     (base << 17) + 2^17 - step - (i << 16)   (mod 2^32)
+ * (base: the byte at the bucket's first pattern; step: the offset of the
+ * pattern where the byte becomes base + 1, or 2^16 if none), so that
+ * (entry + u) >> 17 is the byte for every t of the bucket.  The table is
+ * built on the host with the C library's correctly rounded sqrtf and is
+ * checked against the sqrt sequence for every f32 in [0, 1] by the device
+ * self-test. */
+constexpr uint32_t kQLo = 0x37800000u;           /* below sqrt(t) * 255 >= 1 */
+constexpr int kQBuckets = 2049;                  /* up to bits(1.0f) = 0x3F800000 */
+__device__ uint32_t g_qtab[kQBuckets];
+
+/* (qt entry + u) for t = m1 * s1 + m2 * s2: the channel byte sits in bits
+ * 17..24.  Needs t in [0, 1]. */
+__device__ __forceinline__ uint32_t mix_q(const uint32_t *qt, float m1, float s1, float m2,
+                                          float s2)
+{
+    float t = (m1 * s1) + (m2 * s2);
+    uint32_t u = __builtin_elementwise_sub_sat(__float_as_uint(t), kQLo);
+    return qt[u >> 16] + u;
+}
+
+/* color_fast with the table quantisation (qt: the bucket table in LDS). */
+__device__ __forceinline__ uint32_t color_fast_q(uint4 q, const float *sq, const uint32_t *qt,
+                                                 float wx0, float wx1, float wy0, float wy1)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int sh = 16; sh >= 0; sh -= 8) {
+        uint32_t X1 = mix_q(qt, wx0, sq8(sq, (q.x >> sh) & 255u), wx1,
+                            sq8(sq, (q.y >> sh) & 255u)) >> 17;
+        uint32_t X2 = mix_q(qt, wx0, sq8(sq, (q.z >> sh) & 255u), wx1,
+                            sq8(sq, (q.w >> sh) & 255u)) >> 17;
+        out |= (mix_q(qt, wy0, sq8(sq, X1), wy1, sq8(sq, X2)) >> 17) << sh;
+    }
+    bool opaque_sum = (wx0 + wx1 != 0.0f) && (wy0 + wy1 != 0.0f);
+    return out | (opaque_sum ? 0xFF000000u : 0u);
+}
+
 __device__ __forceinline__ uint32_t color_fast(uint4 q, const float *sq, float wx0, float wx1,
                                                float wy0, float wy1)
 {
@@ -359,15 +450,19 @@ __device__ __forceinline__ uint4 color_quad(const fsk_map &m, float x, float y)
     return ((const uint4 *)m.cq)[(size_t)iy * (size_t)m.ccols + (size_t)ix];
 }
 
-/* The colour of a sample at (x, y) from its texel quad q. */
-__device__ __forceinline__ uint32_t color_of_quad(uint4 q, const float *sq, float x, float y)
+/* The colour of a sample at (x, y) from its texel quad q; with qt (the
+ * quantisation table in LDS) the fast path quantises by table. */
+template <bool QT = false>
+__device__ __forceinline__ uint32_t color_of_quad(uint4 q, const float *sq, float x, float y,
+                                                  const uint32_t *qt = nullptr)
 {
     float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
     float wx0 = cx - x, wx1 = x - fx, wy0 = cy - y, wy1 = y - fy;
     bool easy = (fabsf(x) >= 1.0f) && (fabsf(y) >= 1.0f) &&
                 (((q.x & q.y & q.z & q.w) >> 24) == 255u);
     if (__ballot(!easy) == 0)
-        return color_fast(q, sq, wx0, wx1, wy0, wy1);
+        return QT ? color_fast_q(q, sq, qt, wx0, wx1, wy0, wy1)
+                  : color_fast(q, sq, wx0, wx1, wy0, wy1);
     float mx = wx0 + wx1, my = wy0 + wy1;
     bool odd = !((mx == 1.0f || mx == 0.0f) && (my == 1.0f || my == 0.0f));
     float nx0 = wx0, nx1 = wx1, ny0 = wy0, ny1 = wy1;
@@ -407,18 +502,18 @@ constexpr int kShortSpan = 40;   /* span rows a lane paints itself */
 constexpr int kGather = 2;       /* height gathers a lane keeps in flight */
 constexpr int kColourBatch = 1;  /* visible samples a lane colours per pass */
 
-/* Dynamic LDS of a tile render: slice table (float4 + inv_z per slot),
- * the TC x rows colour stripe, the visible-sample queue and span
- * bottoms (at most min(rows, chunk) samples per column and chunk) and the
- * long-span list. */
-template <int TC>
-size_t render_lds_bytes_tc(int32_t rows, int32_t chunk)
+/* Dynamic LDS of a tile render of tc columns by nt threads: slice table
+ * (float4 + inv_z per slot), the tc x rows colour stripe, the
+ * visible-sample queue (at most min(rows, chunk) samples per column and
+ * chunk), the y-buffer in front of each lane's slices (u16 per thread) and
+ * the long-span list. */
+inline size_t render_lds_bytes(int tc, int nt, int32_t rows, int32_t chunk)
 {
     size_t hp = (size_t)((rows + 3) & ~3);
-    size_t q = (size_t)TC * (size_t)(rows < chunk ? rows : chunk);
-    size_t longs = (size_t)TC * (size_t)(rows / kShortSpan + 1);
+    size_t q = (size_t)tc * (size_t)(rows < chunk ? rows : chunk);
+    size_t longs = (size_t)tc * (size_t)(rows / kShortSpan + 1);
     return sizeof(float) * 5 * (size_t)tab_stride(chunk) +
-           sizeof(uint32_t) * (hp * TC + q + (q + 1) / 2 + 2 * longs);
+           sizeof(uint32_t) * (hp * tc + q + (size_t)nt / 2 + 2 * longs);
 }
 
 /* One workgroup renders a stripe of TC adjacent columns over all slices,
@@ -441,11 +536,22 @@ size_t render_lds_bytes_tc(int32_t rows, int32_t chunk)
 template <int TC>
 struct TileShared {
     float sq[256];                 /* squares table, filled once per workgroup */
+    uint32_t qt[kQBuckets];        /* byte quantisation table (g_qtab) */
     int32_t wtot[kMaxWaves][TC];   /* wave totals of the prefix-min */
     int32_t wpre[kMaxWaves][TC];   /* wave prefixes of the transparent fill */
     int32_t carry[TC];
     uint32_t qn, nlong, transparent;
 };
+
+/* The colour tables of a tile workgroup: squares and the byte
+ * quantisation table (copied from g_qtab). */
+template <int TC, int NT>
+__device__ __forceinline__ void fill_tile_tables(TileShared<TC> &sh, int32_t tid)
+{
+    fill_square_lut(sh.sq, tid);
+    for (int32_t i = tid; i < kQBuckets; i += NT)
+        sh.qt[i] = g_qtab[i];
+}
 
 /* What a tile writes per pixel: the u32 colour (the reference frame) or,
  * for the differentiable renderer, the index of the depth slice whose
@@ -504,8 +610,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     uint32_t *stripe = lds + 5 * tabs;                           /* [TC][Hp] */
     const int32_t qcap = TC * min(H, chunk_max);                 /* visible rows differ */
     uint32_t *queue = stripe + (size_t)Hp * TC;                  /* [qcap] */
-    uint16_t *qbot = (uint16_t *)(queue + qcap);                 /* [qcap] span bottoms */
-    uint32_t *longs = queue + qcap + (qcap + 1) / 2;             /* [2][TC*(H/kShortSpan+1)] */
+    uint16_t *lanein = (uint16_t *)(queue + qcap);               /* [NT] y-buffer per lane */
+    uint32_t *longs = queue + qcap + NT / 2;                     /* [2][TC*(H/kShortSpan+1)] */
 
     const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t col0 = tile * TC;
@@ -535,11 +641,21 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
          * slice).  Computing them per tile is cheaper than copying a
          * precomputed per-view table, even in batches. */
         const int32_t nslots = ((kr + GATHER - 1) / GATHER) * GATHER * SG;
-        for (int32_t o = tid; o < nslots; o += NT) {
-            const int32_t s = min((o % SG) * kr + o / SG, cn - 1);
-            Slice sl = slice_params(f, base + s);
-            tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
-            tz[o] = sl.iz;
+        if (f.tame) {
+            const float rfw = 1.0f / f.fw;
+            for (int32_t o = tid; o < nslots; o += NT) {
+                const int32_t s = min((o % SG) * kr + o / SG, cn - 1);
+                Slice sl = slice_params_tame(f, base + s, rfw);
+                tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
+                tz[o] = sl.iz;
+            }
+        } else {
+            for (int32_t o = tid; o < nslots; o += NT) {
+                const int32_t s = min((o % SG) * kr + o / SG, cn - 1);
+                Slice sl = slice_params(f, base + s);
+                tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
+                tz[o] = sl.iz;
+            }
         }
         if (tid == 0)
             sh.qn = 0u;
@@ -647,9 +763,11 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
             run = min(run, sh.wtot[w][c]);
 
         /* phase 3: mark this lane's visible samples, reserve queue slots
-         * with one wave-level prefix sum, then queue each sample with the
-         * bottom of its span -- the y-buffer just before it, i.e. the next
-         * visible row below in its column */
+         * with one wave-level prefix sum, then queue them in slice order.
+         * A lane's samples take consecutive slots, so the bottom of a
+         * sample's span (the y-buffer just before it: the next visible row
+         * below in its column) is the row of the entry before it, or, for
+         * the lane's first, the y-buffer in front of the lane (lanein) */
         const int32_t run_in = run;
         uint32_t vbits = 0u;
 #pragma unroll
@@ -674,18 +792,16 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 b0 = atomicAdd(&sh.qn, total);
             b0 = __builtin_amdgcn_readfirstlane(b0);
             uint32_t slot = b0 + incl - cnt;
-            int32_t prev = run_in;
 #pragma unroll
             for (int j = 0; j < KR; j++) {
                 if (vbits & (1u << j)) {
                     int32_t aj = (int32_t)((ah[j / 2] >> ((j & 1) * 16)) & 0xFFFFu);
                     queue[slot] = ((uint32_t)g << 21) | ((uint32_t)j << 16) |
                                   ((uint32_t)c << 12) | (uint32_t)aj;
-                    qbot[slot] = (uint16_t)prev;
-                    prev = aj;
                     slot++;
                 }
             }
+            lanein[tid] = (uint16_t)run_in;
         }
         /* the last slice group of each column holds the running minimum of
          * the whole chunk: the sh.carry into the next chunk */
@@ -729,7 +845,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 const float x = xs[u], y = ys[u];
                 uint32_t colour;
                 if (MODE == kOutColour)
-                    colour = color_of_quad(cqs[u], sh.sq, x, y);
+                    colour = color_of_quad<true>(cqs[u], sh.sq, x, y, sh.qt);
                 else if (MODE == kOutNearest)
                     colour = ((const uint32_t *)m.cq)[nearest_index<HP2>(m, x, y)] ^ f.sky;
                 else
@@ -739,7 +855,12 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                     sh.transparent = 1u;
                     continue;
                 }
-                int32_t bottom = (int32_t)qbot[i];
+                /* the entry before, if it is this lane's (same g and c),
+                 * holds the row where this sample's span ends */
+                const uint32_t ep = queue[i > 0 ? i - 1u : 0u];
+                const bool first = i == 0 || ((e ^ ep) & 0x1FE0F000u) != 0;
+                int32_t bottom = first ? (int32_t)lanein[(int32_t)(e >> 21) * TC + cc]
+                                       : (int32_t)(ep & 0xFFFu);
                 uint32_t *dst = stripe + cc * Hp;
                 int32_t stop = min(bottom, row + kShortSpan);
                 for (int32_t r = row; r < stop; r++)
@@ -887,7 +1008,8 @@ __global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
             return;
         }
     }
-    fill_square_lut(sh.sq, threadIdx.x);
+    if (MODE == kOutColour)
+        fill_tile_tables<TC, NT>(sh, threadIdx.x);
     render_tile<TC, NT, CMAX, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
                                                      xcd_tile(blockIdx.x, a.ntiles), a.chunk_max,
                                                      lds, sh);
@@ -1032,16 +1154,38 @@ __global__ void k_shade(const uint16_t *__restrict__ hp, int32_t hrows, int32_t 
  * 2: chan_u8(v) == u32.f32(clamp_channel(v) * 255) with the if-chain
  *    clamp, for every f32 bit pattern.
  * 3: screen_row_clamped(rel, H) == min(screen_row(rel), H) for every f32
- *    bit pattern at H = 1080. */
+ *    bit pattern at H = 1080.
+ * 4: the quantisation table (g_qtab) gives u32.f32(sqrt(t) * 255), with
+ *    the correctly rounded sqrt, for every f32 t in [0, 1].
+ * 5: rcp_rn(z) == 1 / z (correctly rounded) for every f32 z in
+ *    [2^-126, 2^126).
+ * 6: div_rn(a, w, RN(1 / w)) == a / w for every finite f32 a and every
+ *    frame width w of kSelftestWidths. */
+constexpr int32_t kSelftestWidths[] = {1920, 1280, 800, 640, 320, 3840, 128, 96, 1024,
+                                       2048, 4096, 240, 1, 3, 7, 1000, 1366};
+constexpr int kNSelftestWidths = sizeof kSelftestWidths / sizeof kSelftestWidths[0];
 __global__ void k_selftest(int which, unsigned long long *bad)
 {
-    uint64_t n = which == 0 ? 256ull : which == 1 ? 0x3F800001ull : 0x100000000ull;
+    uint64_t n = which == 0 ? 256ull
+                 : (which == 1 || which == 4) ? 0x3F800001ull
+                 : which == 5 ? 0x7E000000ull
+                 : which == 6 ? (uint64_t)kNSelftestWidths << 32
+                              : 0x100000000ull;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long local = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         uint32_t u = (uint32_t)i;
         bool ok = true;
-        if (which == 0) {
+        if (which == 5) {
+            u += 0x00800000u;
+            const float z = __uint_as_float(u);
+            ok = __float_as_uint(rcp_rn(z)) == __float_as_uint(1.0f / z);
+        } else if (which == 6) {
+            const float w = (float)kSelftestWidths[i >> 32];
+            const float a = __uint_as_float(u);
+            if ((u & 0x7F800000u) != 0x7F800000u)
+                ok = __float_as_uint(div_rn(a, w, 1.0f / w)) == __float_as_uint(a / w);
+        } else if (which == 0) {
             ok = __float_as_uint(unit8(u)) == __float_as_uint((float)u / 255.0f);
         } else if (which == 1) {
             float t = __uint_as_float(u);
@@ -1054,9 +1198,13 @@ __global__ void k_selftest(int which, unsigned long long *bad)
             float sc = c * 255.0f;
             uint32_t want = (sc == sc) ? (uint32_t)sc : 0u;
             ok = chan_u8(v) == want;
-        } else {
+        } else if (which == 3) {
             float rel = __uint_as_float(u);
             ok = screen_row_clamped(rel, 1080.0f) == min(screen_row(rel), 1080);
+        } else {
+            float t = __uint_as_float(u);
+            const uint32_t want = (uint32_t)(__builtin_sqrtf(t) * 255.0f);
+            ok = (mix_q(g_qtab, 1.0f, t, 0.0f, 0.0f) >> 17) == want;
         }
         if (!ok) {
             local += 1;
@@ -1089,7 +1237,7 @@ hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
             return e;
         dyn_max = want;
     }
-    size_t need = render_lds_bytes_tc<TC>(a.f.rows, a.chunk_max);
+    size_t need = render_lds_bytes(TC, NT, a.f.rows, a.chunk_max);
     if (need > (size_t)dyn_max)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(kern, dim3(a.ntiles + a.nzero), dim3(NT), need, s, a);
@@ -1156,7 +1304,7 @@ struct Shape {
     int tc, nt, wg, chunk, cmax;
 };
 
-constexpr size_t kStaticLds = 4096;   /* headroom for the kernels' static shared memory */
+constexpr size_t kStaticLds = 12288;  /* headroom for the kernels' static shared memory */
 
 inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
 {
@@ -1165,13 +1313,13 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
     int chunk = (t.chunk >= 256 && t.chunk <= kChunk) ? t.chunk : 0;
     int c8 = chunk && chunk <= 512 ? chunk : 512;
     if (t.tc != 4 && t.wg_per_cu == 2 &&
-        2 * (render_lds_bytes_tc<8>(rows, c8) + kStaticLds) <= lds_per_cu()) {
+        2 * (render_lds_bytes(8, 512, rows, c8) + kStaticLds) <= lds_per_cu()) {
         s.tc = 8;
         s.nt = 512;
         s.wg = 2;
         s.chunk = c8;
         s.cmax = 512;
-    } else if (t.tc != 4 && render_lds_bytes_tc<8>(rows, chunk ? chunk : kChunk) + kStaticLds <=
+    } else if (t.tc != 4 && render_lds_bytes(8, 1024, rows, chunk ? chunk : kChunk) + kStaticLds <=
                                 lds_per_cu()) {
         s.tc = 8;
         s.nt = 1024;
@@ -1181,7 +1329,7 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
         s.tc = 4;
         int c2 = chunk ? chunk : 1024;
         c2 -= c2 % 128;
-        if (t.wg_per_cu == 2 && 2 * (render_lds_bytes_tc<4>(rows, c2) + kStaticLds) <=
+        if (t.wg_per_cu == 2 && 2 * (render_lds_bytes(4, 512, rows, c2) + kStaticLds) <=
                                     lds_per_cu()) {
             s.nt = 512;
             s.wg = 2;
@@ -1197,8 +1345,7 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
     int ga = (s.nt / s.tc) * kChunkAlign;
     s.chunk = s.chunk < ga ? ga : s.chunk - s.chunk % ga;
     auto need = [&](int c) {
-        return (s.tc == 8 ? render_lds_bytes_tc<8>(rows, c) : render_lds_bytes_tc<4>(rows, c)) +
-               kStaticLds;
+        return render_lds_bytes(s.tc, s.nt, rows, c) + kStaticLds;
     };
     while (s.chunk > ga && (size_t)s.wg * need(s.chunk) > lds_per_cu())
         s.chunk = (s.chunk / 2) - (s.chunk / 2) % ga;
@@ -1262,7 +1409,7 @@ __global__ __launch_bounds__(NT, WPE) void k_render_batch(BatchArgs b)
     extern __shared__ uint32_t lds[];
     __shared__ TileShared<TC> sh;
     __shared__ uint32_t cur;
-    fill_square_lut(sh.sq, threadIdx.x);
+    fill_tile_tables<TC, NT>(sh, threadIdx.x);
     uint32_t q = xcc_id() % (uint32_t)b.nqueues;
     int32_t tried = 0;
     for (;;) {
@@ -1306,7 +1453,7 @@ hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
             return e;
         dyn_max = want;
     }
-    size_t need = render_lds_bytes_tc<TC>(fb->max_rows, chunk);
+    size_t need = render_lds_bytes(TC, NT, fb->max_rows, chunk);
     if (need > (size_t)dyn_max)
         return hipErrorInvalidValue;
     BatchArgs b;
@@ -1880,9 +2027,71 @@ inline bool pow2(int32_t v)
     return v > 0 && (v & (v - 1)) == 0;
 }
 
+/* Host: u32.f32(sqrtf(t) * 255) for the f32 with bits tb (sqrtf is
+ * correctly rounded, no contraction: this file is built with
+ * -ffp-contract=off). */
+uint32_t q_exact(uint32_t tb)
+{
+    float t;
+    memcpy(&t, &tb, sizeof t);
+    return (uint32_t)(sqrtf(t) * 255.0f);
+}
+
+/* The bucket table of mix_q (see kQLo): base byte per bucket and the offset
+ * of its single step, found by bisection. */
+void qtab_build(uint32_t *qt)
+{
+    for (int i = 0; i < kQBuckets; i++) {
+        const uint32_t lo = kQLo + ((uint32_t)i << 16);
+        const uint32_t hi = std::min(lo + 0xFFFFu, 0x3F800000u);
+        const uint32_t base = q_exact(lo);
+        uint32_t step = 0x10000u;
+        if (q_exact(hi) != base) {
+            uint32_t a = lo, b = hi;          /* q(a) == base < q(b) */
+            while (b - a > 1) {
+                const uint32_t m = a + (b - a) / 2;
+                if (q_exact(m) > base)
+                    b = m;
+                else
+                    a = m;
+            }
+            step = b - lo;
+        }
+        qt[i] = (base << 17) + 0x20000u - step - ((uint32_t)i << 16);
+    }
+}
+
+std::mutex g_init_mu;
+bool g_init_done[64];
+
 }  // namespace
 
 extern "C" {
+
+int fsk_device_init(void)
+{
+    static uint32_t host_qt[kQBuckets];
+    static bool built = false;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= 64)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_init_mu);
+    if (!built) {
+        qtab_build(host_qt);
+        built = true;
+    }
+    if (!g_init_done[dev]) {
+        e = hipMemcpyToSymbol(HIP_SYMBOL(g_qtab), host_qt, sizeof host_qt, 0,
+                              hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return e;
+        g_init_done[dev] = true;
+    }
+    return hipSuccess;
+}
 
 int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t)
 {
@@ -2210,7 +2419,10 @@ int32_t fsk_frame_tame(const fsk_frame *f)
     const double ymax = (std::fabs((double)f->cam_y) + (ay + (ay + by) * span) * zmax) * margin;
     const double relmax =
         ((std::fabs((double)f->cam_h) + 65536.0) * izmax + std::fabs((double)f->horizon)) * margin;
-    return xmax < 4194304.0 && ymax < 4194304.0 && relmax < 1073741824.0;
+    /* slice depths inside [2^-100, 2^100], where the table's reciprocal
+     * (rcp_rn) is exact */
+    const bool zok = (double)f->z0 >= 0x1p-100 && zmax <= 0x1p100;
+    return zok && xmax < 4194304.0 && ymax < 4194304.0 && relmax < 1073741824.0;
 }
 
 int32_t fsk_render_max_rows(void)
@@ -2220,7 +2432,7 @@ int32_t fsk_render_max_rows(void)
 
 int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s)
 {
-    if (which < 0 || which > 3)
+    if (which < 0 || which > 6)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_selftest, dim3(4096), dim3(256), 0, s, which, d_bad);
     return hipGetLastError();

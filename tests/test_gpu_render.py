@@ -153,11 +153,13 @@ def test_render_into_torch_tensor(ctx, small_scene):
     assert not got[:, 128:].any(), "render wrote past the frame width"
 
 
-@pytest.mark.parametrize("which", [0, 1, 2, 3])
+@pytest.mark.parametrize("which", [0, 1, 2, 3, 4, 5, 6])
 def test_device_arithmetic_selftest(ctx, which):
     """The kernels' shortcuts (byte/255 by reciprocal + FMA correction, sqrt
-    on [0,1] without scaling, median-based quantisation and row clamp) agree
-    with the plain formulas on every input they can see."""
+    on [0,1] without scaling, median-based quantisation, row clamp, the
+    bucket-table quantisation of mixed channels, and the slice table's
+    Newton reciprocal and Markstein divisions by the frame width) agree with
+    the correctly rounded formulas on every input they can see."""
     import ctypes
     bad, first = ctypes.c_int64(-1), ctypes.c_int64(-1)
     ctx.check(ctx.lib.fsx_selftest(ctx.ptr, which, ctypes.byref(bad), ctypes.byref(first)),
