@@ -384,8 +384,8 @@ __device__ __forceinline__ uint32_t mix_channel_exact(float m1, float s1, float 
  * built on the host with the C library's correctly rounded sqrtf and is
  * checked against the sqrt sequence for every f32 in [0, 1] by the device
  * self-test. */
-constexpr uint32_t kQLo = 0x37800000u;           /* below sqrt(t) * 255 >= 1 */
-constexpr int kQBuckets = 2049;                  /* up to bits(1.0f) = 0x3F800000 */
+constexpr uint32_t kQLo = 0x37810000u;           /* below sqrt(t) * 255 >= 1 */
+constexpr int kQBuckets = 2048;                  /* (bits(1.0f) - kQLo) >> 16 = 2047 */
 __device__ uint32_t g_qtab[kQBuckets];
 
 /* (qt entry + u) for t = m1 * s1 + m2 * s2: the channel byte sits in bits
@@ -544,14 +544,33 @@ struct TileShared {
 };
 
 /* The colour tables of a tile workgroup: squares and the byte
- * quantisation table (copied from g_qtab). */
+ * quantisation table, copied from g_qtab in two halves -- load() issues
+ * this thread's loads, store() writes them to LDS -- so that the loads'
+ * latency passes under other work of the tile's first phase. */
 template <int TC, int NT>
-__device__ __forceinline__ void fill_tile_tables(TileShared<TC> &sh, int32_t tid)
-{
-    fill_square_lut(sh.sq, tid);
-    for (int32_t i = tid; i < kQBuckets; i += NT)
-        sh.qt[i] = g_qtab[i];
-}
+struct TileTables {
+    static_assert(kQBuckets % NT == 0, "table split");
+    uint32_t qv[kQBuckets / NT];
+    __device__ __forceinline__ void load(int32_t tid)
+    {
+#pragma unroll
+        for (int k = 0; k < kQBuckets / NT; k++)
+            qv[k] = g_qtab[tid + k * NT];
+    }
+    __device__ __forceinline__ void store(TileShared<TC> &sh, int32_t tid) const
+    {
+        fill_square_lut(sh.sq, tid);
+#pragma unroll
+        for (int k = 0; k < kQBuckets / NT; k++)
+            sh.qt[tid + k * NT] = qv[k];
+    }
+};
+
+/* No colour tables (tile modes without the argb colour path). */
+struct NoTables {
+    template <int TC>
+    __device__ __forceinline__ void store(TileShared<TC> &, int32_t) const {}
+};
 
 /* What a tile writes per pixel: the u32 colour (the reference frame) or,
  * for the differentiable renderer, the index of the depth slice whose
@@ -591,10 +610,11 @@ __device__ __forceinline__ uint32_t nearest_index(const fsk_map &m, float x, flo
 /* chunk_max must be a multiple of NT/TC (the slice groups) and at most
  * CMAX; the LDS layout is sized for it. */
 template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int MODE = kOutColour,
-          int GATHER = kGather, int CBATCH = kColourBatch>
+          int GATHER = kGather, int CBATCH = kColourBatch, class Tables = NoTables>
 __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f, uint32_t *out,
                                             int64_t pitch, int32_t tile, int32_t chunk_max,
-                                            uint32_t *lds, TileShared<TC> &sh)
+                                            uint32_t *lds, TileShared<TC> &sh,
+                                            const Tables &tables = Tables())
 {
     constexpr int NW = NT / 64;                  /* waves */
     constexpr int SG = NT / TC;                  /* slice groups */
@@ -621,8 +641,10 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     const float fcol = (float)col;
     const float Hf = (float)H;
 
-    for (int32_t i = tid; i < Hp * TC; i += NT)
-        stripe[i] = 0u;
+    /* The stripe is not cleared: the spans of a column's visible samples
+     * tile the rows from its final y-buffer down to H (transparent samples
+     * paint 0), and the rows above it are sky, which the write phase tells
+     * apart by the y-buffer. */
     if (tid < TC)
         sh.carry[tid] = H;
     if (tid == 0) {
@@ -659,6 +681,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         }
         if (tid == 0)
             sh.qn = 0u;
+        if (base == 0)
+            tables.store(sh, tid);
         __syncthreads();
 
         /* phase 1: rows are < 2^16, two per register.  Branch-free in
@@ -851,10 +875,10 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 else
                     colour = (floorf(x) == ceilf(x) || floorf(y) == ceilf(y))
                                  ? 0u : (uint32_t)(base + s + 1);
-                if (colour == 0u) {
+                /* a transparent sample paints 0 (fill_vline takes the
+                 * colour above it) */
+                if (colour == 0u)
                     sh.transparent = 1u;
-                    continue;
-                }
                 /* the entry before, if it is this lane's (same g and c),
                  * holds the row where this sample's span ends */
                 const uint32_t ep = queue[i > 0 ? i - 1u : 0u];
@@ -892,19 +916,25 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 
     const bool vec = (col0 + TC <= f.cols) && ((pitch & 3) == 0) &&
                      ((((uintptr_t)out) & 15u) == 0);
+    /* a cell above its column's final y-buffer was never painted: 0 */
+    auto cell = [&](int32_t cc, int32_t r, int32_t ybuf) -> uint32_t {
+        return r < ybuf ? 0u : stripe[cc * Hp + r];
+    };
     if (sh.transparent == 0u) {
-        /* phase 5: every non-zero cell is final; 0 is sky.  TC/4 adjacent
-         * lanes share a row, so each store instruction writes whole
-         * TC*4-byte row segments */
+        /* phase 5: every painted cell is final, the rest is sky.  TC/4
+         * adjacent lanes share a row, so each store instruction writes
+         * whole TC*4-byte row segments */
         constexpr int PARTS = TC / 4;
         const int32_t part = tid % PARTS;
+        int32_t yb[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            yb[j] = sh.carry[part * 4 + j];
         for (int32_t r = tid / PARTS; r < H; r += NT / PARTS) {
             uint32_t px[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                uint32_t s = stripe[(part * 4 + j) * Hp + r];
-                px[j] = tile_value<MODE>(s, f.sky);
-            }
+            for (int j = 0; j < 4; j++)
+                px[j] = tile_value<MODE>(cell(part * 4 + j, r, yb[j]), f.sky);
             uint32_t *dst = out + (size_t)r * (size_t)pitch + col0 + part * 4;
             if (vec) {
                 *(uint4 *)dst = make_uint4(px[0], px[1], px[2], px[3]);
@@ -924,11 +954,13 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     const int32_t per = (H + NT - 1) / NT;
     const int32_t r0 = min(tid * per, H), r1 = min(r0 + per, H);
     uint32_t pre[TC];
+    int32_t yb[TC];
 #pragma unroll
     for (int j = 0; j < TC; j++) {
+        yb[j] = sh.carry[j];
         uint32_t p = 0u;
         for (int32_t r = r0; r < r1; r++) {
-            uint32_t s = stripe[j * Hp + r];
+            uint32_t s = cell(j, r, yb[j]);
             p = s ? s : p;
         }
         pre[j] = p;
@@ -976,7 +1008,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         uint32_t px[TC];
 #pragma unroll
         for (int j = 0; j < TC; j++) {
-            uint32_t s = stripe[j * Hp + r];
+            uint32_t s = cell(j, r, yb[j]);
             pre[j] = s ? s : pre[j];
             px[j] = tile_value<MODE>(pre[j], f.sky);
         }
@@ -997,7 +1029,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
 __global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];  /* float4 slice table at its start */
     __shared__ TileShared<TC> sh;
     if constexpr (MODE == kOutSlice) {
         if ((int32_t)blockIdx.x >= a.ntiles) {
@@ -1008,11 +1040,17 @@ __global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
             return;
         }
     }
-    if (MODE == kOutColour)
-        fill_tile_tables<TC, NT>(sh, threadIdx.x);
-    render_tile<TC, NT, CMAX, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
-                                                     xcd_tile(blockIdx.x, a.ntiles), a.chunk_max,
-                                                     lds, sh);
+    if constexpr (MODE == kOutColour) {
+        TileTables<TC, NT> tables;
+        tables.load(threadIdx.x);
+        render_tile<TC, NT, CMAX, HBITS, HP2, CP2, MODE, kGather, kColourBatch,
+                    TileTables<TC, NT>>(a.m, a.f, a.out, a.pitch, xcd_tile(blockIdx.x, a.ntiles),
+                                        a.chunk_max, lds, sh, tables);
+    } else {
+        render_tile<TC, NT, CMAX, HBITS, HP2, CP2, MODE>(a.m, a.f, a.out, a.pitch,
+                                                         xcd_tile(blockIdx.x, a.ntiles),
+                                                         a.chunk_max, lds, sh);
+    }
 }
 
 /* render2 (fut/voxel_renderer.fut:257-305), the reference's ray-march
@@ -1406,10 +1444,14 @@ __global__ void k_batch_reset(uint32_t *counters, int32_t n)
 template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE>
 __global__ __launch_bounds__(NT, WPE) void k_render_batch(BatchArgs b)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];  /* float4 slice table at its start */
     __shared__ TileShared<TC> sh;
     __shared__ uint32_t cur;
-    fill_tile_tables<TC, NT>(sh, threadIdx.x);
+    {
+        TileTables<TC, NT> tables;
+        tables.load(threadIdx.x);
+        tables.store(sh, threadIdx.x);    /* the claim loop's barrier publishes them */
+    }
     uint32_t q = xcc_id() % (uint32_t)b.nqueues;
     int32_t tried = 0;
     for (;;) {
