@@ -108,6 +108,7 @@ FUTSPACE_SIGNATURES = {
     "fsx_batch_replay": (ctypes.c_int, [P, P]),
     "fsx_batch_free": (ctypes.c_int, [P, P]),
     "fsx_set_tuning": (ctypes.c_int, [P, I32, I32, I32]),
+    "fsx_get_tuning": (ctypes.c_int, [P, P]),
     "fsx_context_set_stream": (ctypes.c_int, [P, P]),
     "fsx_context_stream": (P, [P]),
     "fsx_map_shade_factor": (ctypes.c_int, [P, P, P]),

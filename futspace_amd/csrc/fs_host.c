@@ -61,6 +61,15 @@ struct futhark_context {
 };
 
 #define FS_PARTIALS 32768     /* floats in d_partials */
+#define FS_STEP_BLOCK_COLS 64 /* pixel blocks of the colour / loss pass and */
+#define FS_STEP_BLOCK_ROWS 16 /* of the adjoint: one partial sum per block */
+
+/* Pixel blocks of a w x h frame in the differentiable passes. */
+static int64_t step_blocks(int32_t w, int32_t h)
+{
+    return (int64_t)((w + FS_STEP_BLOCK_COLS - 1) / FS_STEP_BLOCK_COLS) *
+           (int64_t)((h + FS_STEP_BLOCK_ROWS - 1) / FS_STEP_BLOCK_ROWS);
+}
 
 #define FS_PROF_MAX 4096
 
@@ -263,6 +272,7 @@ struct futhark_context *futhark_context_new(struct futhark_context_config *cfg)
         return NULL;
     }
     ctx->stream = ctx->own_stream;
+    fsk_default_tuning(&ctx->tune);
     if (fsk_device_init() != hipSuccess) {
         fprintf(stderr, "futspace: device table upload failed\n");
         hipStreamDestroy(ctx->own_stream);
@@ -866,7 +876,7 @@ static int render_into(struct futhark_context *ctx, const struct fs_terrain *t,
     m.ccols = s->cols;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
-    HIPCHECK(ctx, fsk_render(&m, &f, d_out, pitch > 0 ? pitch : v->w, ctx->stream));
+    HIPCHECK(ctx, fsk_render(&m, &f, &ctx->tune, d_out, pitch > 0 ? pitch : v->w, ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }
@@ -1566,7 +1576,8 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
     m.ccols = r;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
-    HIPCHECK(ctx, fsk_render_nearest(&m, &f, (uint32_t *)dev_out, view->w, ctx->stream));
+    HIPCHECK(ctx, fsk_render_nearest(&m, &f, &ctx->tune, (uint32_t *)dev_out, view->w,
+                                     ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }
@@ -1641,7 +1652,7 @@ int fsx_render_f32(struct futhark_context *ctx, const struct fsx_map *map,
     m.ccols = wc;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
-    HIPCHECK(ctx, fsk_render_slices(&m, &f, dev_src, view->w, ctx->stream));
+    HIPCHECK(ctx, fsk_render_slices(&m, &f, &ctx->tune, dev_src, view->w, ctx->stream));
     HIPCHECK(ctx, fsk_diff_color(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_img,
                                  ctx->stream));
     prof_end(ctx, pe);
@@ -1662,13 +1673,20 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
         set_error(ctx, "fsx_render_f32_vjp: null buffer");
         return 1;
     }
+    if (step_blocks(view->w, view->h) > FS_PARTIALS) {
+        set_error(ctx, "fsx_render_f32_vjp: a %d x %d frame has %lld blocks of %d x %d pixels, "
+                  "more than the %d partial sums of the adjoint", view->w, view->h,
+                  (long long)step_blocks(view->w, view->h), FS_STEP_BLOCK_COLS,
+                  FS_STEP_BLOCK_ROWS, FS_PARTIALS);
+        return 1;
+    }
+    float *part;
+    if (partials(ctx, &part))
+        return 1;
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
     if (!accumulate)
         HIPCHECK(ctx, fsk_zero_f32(dev_gplanes, (size_t)3 * (size_t)hc * (size_t)wc, ctx->stream));
-    float *part;
-    if (partials(ctx, &part))
-        return 1;
     if (view->h > 0 && view->w > 0)
         HIPCHECK(ctx, fsk_diff_grad(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg,
                                     dev_gplanes, dev_gsky, accumulate != 0, part, FS_PARTIALS,
@@ -1696,6 +1714,14 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
         return 1;
     }
     const size_t nplanes = (size_t)3 * (size_t)hc * (size_t)wc;
+    /* the loss and sky partials take half of the scratch each */
+    if (step_blocks(view->w, view->h) > FS_PARTIALS / 2) {
+        set_error(ctx, "fsx_render_f32_l2_step: a %d x %d frame has %lld blocks of %d x %d "
+                  "pixels, more than the %d loss partial sums of the fused step", view->w,
+                  view->h, (long long)step_blocks(view->w, view->h), FS_STEP_BLOCK_COLS,
+                  FS_STEP_BLOCK_ROWS, FS_PARTIALS / 2);
+        return 1;
+    }
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
     if (view->h == 0 || view->w == 0) {
@@ -1722,22 +1748,22 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
      * render's spare workgroups (CUs its tiles leave idle) or else inside
      * the colour launch */
     int32_t cleared = 0;
-    HIPCHECK(ctx, fsk_render_slices_clear(&m, &f, dev_src, view->w, accumulate ? NULL : dev_gplanes,
-                                          nplanes, &cleared, ctx->stream));
-    /* the forward's loss and sky partials take the scratch's first and
-     * second quarter and are summed by the adjoint's first workgroup (the
-     * sky term of the loss gradient is known once g is); the adjoint's own
-     * partials would take the second half */
-    const int32_t q = FS_PARTIALS / 4;
+    HIPCHECK(ctx, fsk_render_slices_clear(&m, &f, &ctx->tune, dev_src, view->w,
+                                          accumulate ? NULL : dev_gplanes, nplanes, &cleared,
+                                          ctx->stream));
+    /* the forward's loss and sky partials take the scratch's two halves
+     * and are summed by the adjoint's first workgroup (the sky term of the
+     * loss gradient is known once g is); given them, the adjoint writes no
+     * partials of its own */
+    const int32_t q = FS_PARTIALS / 2;
     int32_t nloss = 0;
     HIPCHECK(ctx, fsk_diff_color_l2(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_target,
                                     dev_img, dev_gimg, dev_loss, accumulate != 0,
                                     accumulate || cleared ? NULL : dev_gplanes, nplanes, part, q,
-                                    &nloss,
-                                    part + q, ctx->stream));
+                                    &nloss, part + q, ctx->stream));
     HIPCHECK(ctx, fsk_diff_grad_l2(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg, dev_gplanes,
-                                   dev_gsky, accumulate != 0, part + 2 * q, 2 * q, part, nloss,
-                                   dev_loss, accumulate != 0, part + q, ctx->stream));
+                                   dev_gsky, accumulate != 0, NULL, 0, part, nloss, dev_loss,
+                                   accumulate != 0, part + q, ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }
@@ -1791,7 +1817,15 @@ int fsx_set_tuning(struct futhark_context *ctx, int32_t tile_columns, int32_t wg
     }
     fsk_tuning t = {tile_columns, wg_per_cu, chunk};
     ctx->tune = t;
-    fsk_set_tuning(&t);
+    return 0;
+}
+
+int fsx_get_tuning(struct futhark_context *ctx, int32_t out[3])
+{
+    REQUIRE_ARGS(ctx, out, "fsx_get_tuning");
+    out[0] = ctx->tune.tc;
+    out[1] = ctx->tune.wg_per_cu;
+    out[2] = ctx->tune.chunk;
     return 0;
 }
 

@@ -55,18 +55,23 @@ int fsk_device_init(void);
 int fsk_render2(const fsk_map *m, const fsk_frame *f, const float *d_cs, const float *d_sn,
                 const uint32_t *d_color, int32_t n, uint32_t *d_out, int64_t pitch, hipStream_t s);
 
-/* Plain render of one frame into out[rows][pitch]; the kernel builds its
- * own slice table per depth chunk. */
-int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
-               hipStream_t s);
-int32_t fsk_render_max_rows(void);
-
 /* Tuning knobs of the render kernels (0 = automatic). */
 typedef struct {
     int32_t tc;          /* frame columns per workgroup tile: 4 or 8 */
-    int32_t wg_per_cu;   /* 1 or 2 workgroups per CU (4-column tiles only) */
+    int32_t wg_per_cu;   /* 1 or 2 workgroups per CU */
     int32_t chunk;       /* depth slices per chunk: 256..2048 */
 } fsk_tuning;
+
+/* The tuning a new context starts from: the FSX_RENDER_TC,
+ * FSX_RENDER_WG_PER_CU and FSX_RENDER_CHUNK environment variables (unset:
+ * 0, automatic). */
+void fsk_default_tuning(fsk_tuning *t);
+
+/* Plain render of one frame into out[rows][pitch] in the kernel shape
+ * that t selects; the kernel builds its own slice table per depth chunk. */
+int fsk_render(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t, uint32_t *d_out,
+               int64_t pitch, hipStream_t s);
+int32_t fsk_render_max_rows(void);
 
 /* One view of a batched render, as the persistent kernel reads it. */
 typedef struct {
@@ -94,8 +99,6 @@ int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t);
 int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t);
 /* Reset the claim counters and run the persistent batched render. */
 int fsk_batch_run(const fsk_batch *b, hipStream_t s);
-/* Select the plain render's tuning (process-wide). */
-void fsk_set_tuning(const fsk_tuning *t);
 
 /* Device self-test `which` (0-6) of the arithmetic shortcuts; adds the
  * number of mismatching inputs to *d_bad. */
@@ -116,18 +119,19 @@ int fsk_color_quads(const uint32_t *d_col, int32_t rows, int32_t cols, void *d_c
  * there (-1: sky); fsk_diff_color turns it into img[3][rows][cols];
  * fsk_diff_grad adds dL/dcp into d_gcp[3][hc][wc] and dL/dsky_scale into
  * *d_gsky (both accumulate). */
-int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
-                      hipStream_t s);
+int fsk_render_slices(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t, int32_t *d_src,
+                      int64_t pitch, hipStream_t s);
 /* fsk_render_slices that also clears d_zero[zero_n] (float4-aligned) in
  * extra workgroups on the CUs its tiles leave idle, when enough are; sets
  * *cleared to 1 if it did (else the caller clears). */
-int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
-                            float *d_zero, size_t zero_n, int32_t *cleared, hipStream_t s);
+int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t,
+                            int32_t *d_src, int64_t pitch, float *d_zero, size_t zero_n,
+                            int32_t *cleared, hipStream_t s);
 /* The stateless program of tests/futspace.fut: m.hq = i32 heights and
  * m.cq = i32 colours, both [hrows][hcols], nearest sampling; f.sky is the
  * program's sky (also its `empty` colour). */
-int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
-                       hipStream_t s);
+int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t,
+                       uint32_t *d_out, int64_t pitch, hipStream_t s);
 int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
                    int32_t wc, float sky_scale, const int32_t *d_src, float *d_img, hipStream_t s);
 int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,

@@ -22,6 +22,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -1253,28 +1254,78 @@ __global__ void k_selftest(int which, unsigned long long *bad)
         atomicAdd(bad, local);
 }
 
+/* Per-device caches of device properties and kernel set-up (a process may
+ * drive several devices; each has its own). */
+constexpr int kMaxDevices = 64;
+
+inline int current_device()
+{
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices)
+        d = 0;
+    return d;
+}
+
+inline size_t lds_per_cu()
+{
+    static std::atomic<int> v[kMaxDevices];
+    const int dev = current_device();
+    int x = v[dev].load(std::memory_order_relaxed);
+    if (x == 0) {
+        if (hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+                hipSuccess ||
+            x <= 0)
+            x = 65536;
+        v[dev].store(x, std::memory_order_relaxed);
+    }
+    return (size_t)x;
+}
+
+inline int cu_count()
+{
+    static std::atomic<int> v[kMaxDevices];
+    const int dev = current_device();
+    int x = v[dev].load(std::memory_order_relaxed);
+    if (x == 0) {
+        if (hipDeviceGetAttribute(&x, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            x <= 0)
+            x = 256;
+        v[dev].store(x, std::memory_order_relaxed);
+    }
+    return x;
+}
+
+/* Lets `kern` take all of a CU's LDS but its static shared memory, once per
+ * device; returns that dynamic maximum (0 on failure, with *err set). */
+inline int dyn_lds_max(const void *kern, std::atomic<int> *cache, hipError_t *err)
+{
+    const int dev = current_device();
+    int x = cache[dev].load(std::memory_order_relaxed);
+    if (x > 0)
+        return x;
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, kern);
+    if (e == hipSuccess) {
+        x = (int)lds_per_cu() - (int)fa.sharedSizeBytes;
+        e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, x);
+    }
+    if (e != hipSuccess) {
+        *err = e;
+        return 0;
+    }
+    cache[dev].store(x, std::memory_order_relaxed);
+    return x;
+}
+
 template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
 hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
 {
     auto kern = k_render<TC, NT, CMAX, HBITS, HP2, CP2, WPE, MODE>;
-    static int dyn_max = -1;
-    if (dyn_max < 0) {
-        /* the whole LDS of a CU minus the kernel's static shared memory */
-        int dev = 0, lds = 0;
-        hipFuncAttributes fa;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess)
-            e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
-        if (e == hipSuccess)
-            e = hipFuncGetAttributes(&fa, (const void *)kern);
-        if (e != hipSuccess)
-            return e;
-        int want = lds - (int)fa.sharedSizeBytes;
-        e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, want);
-        if (e != hipSuccess)
-            return e;
-        dyn_max = want;
-    }
+    static std::atomic<int> dyn_cache[kMaxDevices];
+    hipError_t e = hipSuccess;
+    const int dyn_max = dyn_lds_max((const void *)kern, dyn_cache, &e);
+    if (!dyn_max)
+        return e;
     size_t need = render_lds_bytes(TC, NT, a.f.rows, a.chunk_max);
     if (need > (size_t)dyn_max)
         return hipErrorInvalidValue;
@@ -1292,51 +1343,10 @@ hipError_t launch_render_cfg(const RenderArgs &a, hipStream_t s)
  * the other's work -- worth it when there are many more tiles than CUs, as
  * in batches; they need the LDS of two tiles, so their depth chunk is
  * shorter: at most 512 slices for 8 columns, 1024 for 4), and the
- * depth-chunk length.  fsk_set_tuning (the autotuner) or the
- * FSX_RENDER_TC / FSX_RENDER_WG_PER_CU / FSX_RENDER_CHUNK environment
- * variables choose them; 0 means automatic. */
-fsk_tuning g_tuning = {-1, -1, -1};
-
-inline fsk_tuning current_tuning()
-{
-    if (g_tuning.tc < 0) {
-        const char *e = getenv("FSX_RENDER_TC");
-        g_tuning.tc = e ? atoi(e) : 0;
-        e = getenv("FSX_RENDER_WG_PER_CU");
-        g_tuning.wg_per_cu = e ? atoi(e) : 0;
-        e = getenv("FSX_RENDER_CHUNK");
-        g_tuning.chunk = e ? atoi(e) : 0;
-    }
-    return g_tuning;
-}
-
-inline size_t lds_per_cu()
-{
-    static size_t v = 0;
-    if (v == 0) {
-        int dev = 0, lds = 65536;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
-                hipSuccess)
-            lds = 65536;
-        v = (size_t)lds;
-    }
-    return v;
-}
-
-inline int cu_count()
-{
-    static int v = 0;
-    if (v == 0) {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            n <= 0)
-            n = 256;
-        v = n;
-    }
-    return v;
-}
+ * depth-chunk length.  Each context carries its own tuning (fsx_set_tuning,
+ * the autotuner), which starts from the FSX_RENDER_TC /
+ * FSX_RENDER_WG_PER_CU / FSX_RENDER_CHUNK environment variables; 0 means
+ * automatic. */
 
 struct Shape {
     int tc, nt, wg, chunk, cmax;
@@ -1391,9 +1401,10 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
 }
 
 template <int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
-hipError_t launch_render(RenderArgs a, hipStream_t s, int32_t *nzero = nullptr)
+hipError_t launch_render(RenderArgs a, const fsk_tuning &tune, hipStream_t s,
+                         int32_t *nzero = nullptr)
 {
-    Shape sh = resolve_shape(a.f.rows, current_tuning());
+    Shape sh = resolve_shape(a.f.rows, tune);
     a.chunk_max = sh.chunk;
     a.ntiles = (a.f.cols + sh.tc - 1) / sh.tc;
     /* slice mode: the workgroup slots the tiles leave free clear the
@@ -1483,18 +1494,11 @@ template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE>
 hipError_t launch_batch_cfg(const fsk_batch *fb, int chunk, hipStream_t s)
 {
     auto kern = k_render_batch<TC, NT, CMAX, HBITS, HP2, CP2, WPE>;
-    static int dyn_max = -1;
-    if (dyn_max < 0) {
-        hipFuncAttributes fa;
-        hipError_t e = hipFuncGetAttributes(&fa, (const void *)kern);
-        if (e != hipSuccess)
-            return e;
-        int want = (int)lds_per_cu() - (int)fa.sharedSizeBytes;
-        e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, want);
-        if (e != hipSuccess)
-            return e;
-        dyn_max = want;
-    }
+    static std::atomic<int> dyn_cache[kMaxDevices];
+    hipError_t e = hipSuccess;
+    const int dyn_max = dyn_lds_max((const void *)kern, dyn_cache, &e);
+    if (!dyn_max)
+        return e;
     size_t need = render_lds_bytes(TC, NT, fb->max_rows, chunk);
     if (need > (size_t)dyn_max)
         return hipErrorInvalidValue;
@@ -2146,9 +2150,14 @@ int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t)
     return cu_count() * sh.wg;
 }
 
-void fsk_set_tuning(const fsk_tuning *t)
+void fsk_default_tuning(fsk_tuning *t)
 {
-    g_tuning = *t;
+    const char *e = getenv("FSX_RENDER_TC");
+    t->tc = e ? atoi(e) : 0;
+    e = getenv("FSX_RENDER_WG_PER_CU");
+    t->wg_per_cu = e ? atoi(e) : 0;
+    e = getenv("FSX_RENDER_CHUNK");
+    t->chunk = e ? atoi(e) : 0;
 }
 
 int fsk_batch_run(const fsk_batch *b, hipStream_t s)
@@ -2160,8 +2169,9 @@ int fsk_batch_run(const fsk_batch *b, hipStream_t s)
     return b->pow2 ? launch_batch<16, true, true>(b, s) : launch_batch<16, false, false>(b, s);
 }
 
-int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
-                            float *d_zero, size_t zero_n, int32_t *cleared, hipStream_t s)
+int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t,
+                            int32_t *d_src, int64_t pitch, float *d_zero, size_t zero_n,
+                            int32_t *cleared, hipStream_t s)
 {
     if (cleared)
         *cleared = 0;
@@ -2191,20 +2201,20 @@ int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, int32_t *d_src
     bool hp2 = pow2(m->hrows) && pow2(m->hcols);
     hipError_t e;
     if (m->hbits == 8)
-        e = hp2 ? launch_render<8, true, false, kOutSlice>(a, s, &nz)
-                : launch_render<8, false, false, kOutSlice>(a, s, &nz);
+        e = hp2 ? launch_render<8, true, false, kOutSlice>(a, *t, s, &nz)
+                : launch_render<8, false, false, kOutSlice>(a, *t, s, &nz);
     else
-        e = hp2 ? launch_render<16, true, false, kOutSlice>(a, s, &nz)
-                : launch_render<16, false, false, kOutSlice>(a, s, &nz);
+        e = hp2 ? launch_render<16, true, false, kOutSlice>(a, *t, s, &nz)
+                : launch_render<16, false, false, kOutSlice>(a, *t, s, &nz);
     if (e == hipSuccess && cleared)
         *cleared = nz > 0;
     return e;
 }
 
-int fsk_render_slices(const fsk_map *m, const fsk_frame *f, int32_t *d_src, int64_t pitch,
-                      hipStream_t s)
+int fsk_render_slices(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t, int32_t *d_src,
+                      int64_t pitch, hipStream_t s)
 {
-    return fsk_render_slices_clear(m, f, d_src, pitch, nullptr, 0, nullptr, s);
+    return fsk_render_slices_clear(m, f, t, d_src, pitch, nullptr, 0, nullptr, s);
 }
 
 static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, int32_t hc,
@@ -2232,8 +2242,8 @@ static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, 
     return a;
 }
 
-int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
-                       hipStream_t s)
+int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t,
+                       uint32_t *d_out, int64_t pitch, hipStream_t s)
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
@@ -2252,8 +2262,8 @@ int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, in
     a.zero = nullptr;
     a.zero_n4 = 0;
     a.nzero = 0;
-    return pow2(m->hrows) && pow2(m->hcols) ? launch_render<8, true, false, kOutNearest>(a, s)
-                                            : launch_render<8, false, false, kOutNearest>(a, s);
+    return pow2(m->hrows) && pow2(m->hcols) ? launch_render<8, true, false, kOutNearest>(a, *t, s)
+                                            : launch_render<8, false, false, kOutNearest>(a, *t, s);
 }
 
 int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
@@ -2339,7 +2349,9 @@ int fsk_diff_grad_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, i
     a.loss_accumulate = loss_accumulate;
     a.sky_part = (float *)d_sky_part;
     dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
-    if ((int64_t)grid.x * grid.y > n_partials)
+    /* the sums of a fused step's forward (loss_n > 0) leave no partials
+     * of the adjoint's own */
+    if (loss_n == 0 && (int64_t)grid.x * grid.y > n_partials)
         return hipErrorInvalidValue;
     void (*kern)(DiffArgs) = pow2(hc) && pow2(wc) ? k_diff_grad<true> : k_diff_grad<false>;
     hipLaunchKernelGGL(kern, grid, dim3(kGradThreads), 0, s, a);
@@ -2480,8 +2492,8 @@ int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s)
     return hipGetLastError();
 }
 
-int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pitch,
-               hipStream_t s)
+int fsk_render(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t, uint32_t *d_out,
+               int64_t pitch, hipStream_t s)
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
@@ -2504,15 +2516,15 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, uint32_t *d_out, int64_t pi
     bool cp2 = pow2(m->crows) && pow2(m->ccols);
     hipError_t e;
     if (m->hbits == 8) {
-        if (hp2 && cp2) e = launch_render<8, true, true>(a, s);
-        else if (hp2) e = launch_render<8, true, false>(a, s);
-        else if (cp2) e = launch_render<8, false, true>(a, s);
-        else e = launch_render<8, false, false>(a, s);
+        if (hp2 && cp2) e = launch_render<8, true, true>(a, *t, s);
+        else if (hp2) e = launch_render<8, true, false>(a, *t, s);
+        else if (cp2) e = launch_render<8, false, true>(a, *t, s);
+        else e = launch_render<8, false, false>(a, *t, s);
     } else {
-        if (hp2 && cp2) e = launch_render<16, true, true>(a, s);
-        else if (hp2) e = launch_render<16, true, false>(a, s);
-        else if (cp2) e = launch_render<16, false, true>(a, s);
-        else e = launch_render<16, false, false>(a, s);
+        if (hp2 && cp2) e = launch_render<16, true, true>(a, *t, s);
+        else if (hp2) e = launch_render<16, true, false>(a, *t, s);
+        else if (cp2) e = launch_render<16, false, true>(a, *t, s);
+        else e = launch_render<16, false, false>(a, *t, s);
     }
     return e;
 }

@@ -46,10 +46,12 @@ class DiffRenderer:
     @contextlib.contextmanager
     def stream(self):
         """The context's HIP stream, ordered after torch's current stream on
-        entry and before it on exit.  Nested uses order the two streams only
-        at the outermost entry and exit: torch work issued inside must not
-        read this renderer's outputs before the outermost exit."""
+        every entry (so the renderer sees torch's earlier work, also inside
+        an outer block) and before it at the outermost exit: torch work
+        issued inside a block must not read this renderer's outputs before
+        the outermost exit."""
         if self._depth:
+            self._ext.wait_stream(torch.cuda.current_stream())
             self._depth += 1
             try:
                 yield self._ext
@@ -195,18 +197,27 @@ def planes_of(color):
         np.float32(255.0)
 
 
+COLOUR_UNIT = 255.0   # a colour-map channel step (u8) in plane units
+
+
 def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None):
     """Multi-view colour-map fitting: `steps` Adam steps on planes (in
     place) against target frames targets[i] (3 x h x w f32 CUDA tensors),
     loss = sum over views of the squared L2 distance.  Gradients of all
     views accumulate into one buffer through the adjoint's scatter-add.
-    Returns the per-step losses (a CUDA tensor, read once at the end)."""
+
+    lr is in the colour map's own unit, one u8 channel step (the planes hold
+    channel / 255), so Adam moves each plane value by about lr / 255 per
+    step; after each step the planes are clamped back to [0, 1], the domain
+    of the relaxation.  Returns the per-step losses (a CUDA tensor, read
+    once at the end)."""
     planes.requires_grad_(False)
+    step = lr / COLOUR_UNIT
     try:
         # one pass over parameter, gradient and moments per step
-        opt = torch.optim.Adam([planes], lr=lr, fused=True)
+        opt = torch.optim.Adam([planes], lr=step, fused=True)
     except (RuntimeError, TypeError):
-        opt = torch.optim.Adam([planes], lr=lr)
+        opt = torch.optim.Adam([planes], lr=step)
     grad = torch.zeros_like(planes)
     gsky = torch.zeros(1, dtype=torch.float32, device="cuda")
     img = gimg = src = None
@@ -224,6 +235,7 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
                                  losses[it:it + 1], accumulate=i > 0)
         planes.grad = grad
         opt.step()
+        planes.clamp_(0.0, 1.0)
         if on_step is not None:
             on_step(it)
     return losses

@@ -194,9 +194,16 @@ class Context:
         return out.cpu().numpy()
 
     def set_tuning(self, tile_columns=0, wg_per_cu=0, chunk=0):
-        """Render kernel shape (0 = automatic); see fsx_set_tuning."""
+        """Render kernel shape of this context (0 = automatic); see
+        fsx_set_tuning."""
         self.check(self.lib.fsx_set_tuning(self.ptr, tile_columns, wg_per_cu, chunk),
                    "fsx_set_tuning")
+
+    def tuning(self):
+        """(tile_columns, wg_per_cu, chunk) of this context."""
+        out = (ctypes.c_int32 * 3)()
+        self.check(self.lib.fsx_get_tuning(self.ptr, out), "fsx_get_tuning")
+        return tuple(out)
 
     def profile(self, enable=True):
         self.check(self.lib.fsx_profile_enable(self.ptr, 1 if enable else 0),

@@ -225,7 +225,10 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
  * fsx_render_f32_vjp adds dL/d planes into dev_gplanes[3][hc][wc] and
  * dL/d sky_scale into *dev_gsky, given dev_gimg = dL/d img and the src of
  * the forward pass; accumulate = 0 clears both first.  f32 atomics: the
- * summation order, hence the last bits, varies from run to run.
+ * summation order, hence the last bits, varies from run to run.  Its
+ * per-workgroup partial sums limit the frame to ceil(w/64) * ceil(h/16)
+ * <= 32768 blocks of 64 x 16 pixels (h <= 4096 rows: every w up to 8192);
+ * larger frames are refused before anything is enqueued.
  *
  * All three are asynchronous on the context stream. */
 int fsx_map_shade_factor(struct futhark_context *ctx, const struct fsx_map *map, float *dev_sf);
@@ -253,7 +256,10 @@ int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *
  * colour pass does), the colour and loss pass, and the adjoint (which
  * first sums the loss and sky partials).  The loss is summed per workgroup
  * in a fixed order, so the loss of a step is reproducible; the gradients
- * carry the f32-atomic summation order of fsx_render_f32_vjp.
+ * carry the f32-atomic summation order of fsx_render_f32_vjp.  The loss
+ * and sky partial sums (one each per 64 x 16 pixel block) limit the frame
+ * to ceil(w/64) * ceil(h/16) <= 16384 blocks (4096 x 4096, or 8192 columns
+ * by 2048 rows); larger frames are refused before anything is enqueued.
  * Asynchronous on the context stream. */
 int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
                            const struct fsx_view *view, const float *dev_planes,
@@ -265,10 +271,15 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
 /* Render kernel shape, as chosen by the autotuner: columns per workgroup
  * tile (4 or 8), workgroups per CU (1 of 1024 threads, or 2 of 512 whose
  * depth chunks are shortened until two tiles fit in LDS) and depth slices
- * per chunk (256-2048); 0 leaves a knob automatic.  Applies to later
- * renders and to batches prepared afterwards. */
+ * per chunk (256-2048); 0 leaves a knob automatic.  The setting belongs
+ * to this context: it applies to its later renders and to batches it
+ * prepares afterwards, not to other contexts.  A new context starts from
+ * the FSX_RENDER_TC, FSX_RENDER_WG_PER_CU and FSX_RENDER_CHUNK environment
+ * variables (automatic where unset). */
 int fsx_set_tuning(struct futhark_context *ctx, int32_t tile_columns, int32_t wg_per_cu,
                    int32_t chunk);
+/* The context's current tuning, as (tile_columns, wg_per_cu, chunk). */
+int fsx_get_tuning(struct futhark_context *ctx, int32_t out[3]);
 
 /* Use `stream` (a hipStream_t, or NULL for the context's own stream) for
  * all later device work of the context. */

@@ -189,9 +189,31 @@ def test_fit_reduces_loss(ctx):
     tp = torch.from_numpy(target_planes.astype(np.float32)).cuda()
     targets = [r.forward_raw(tp, SUN_H, v)[0].clone() for v in views]
     planes = torch.from_numpy(planes0).cuda()
-    losses = fit(r, planes, views, targets, SUN_H, steps=20, lr=0.01).cpu().numpy()
+    losses = fit(r, planes, views, targets, SUN_H, steps=20, lr=2.0).cpu().numpy()
     assert np.all(np.isfinite(losses))
     assert losses[-1] < 0.5 * losses[0], losses
+    p = planes.cpu().numpy()
+    assert p.min() >= 0.0 and p.max() <= 1.0
+
+
+def test_fit_at_the_workload_rate_does_not_diverge(ctx):
+    """Adam at the fitting workload's lr 0.5 (one half u8 channel step):
+    the loss falls over the first steps instead of blowing up, and the
+    planes stay inside [0, 1]."""
+    r, col, _, _ = _setup(ctx, seed=31, size=128)
+    rng = np.random.default_rng(8)
+    planes0 = planes_of(col)
+    target_planes = np.clip(planes0 + rng.normal(0.0, 8.0 / 255.0, planes0.shape), 0, 1)
+    views = VIEWS[:3]
+    tp = torch.from_numpy(target_planes.astype(np.float32)).cuda()
+    targets = [r.forward_raw(tp, SUN_H, v)[0].clone() for v in views]
+    planes = torch.from_numpy(planes0).cuda()
+    losses = fit(r, planes, views, targets, SUN_H, steps=12, lr=0.5).cpu().numpy()
+    assert np.all(np.isfinite(losses))
+    assert np.all(np.diff(losses) <= 1e-6 * losses[0]), losses
+    assert losses[-1] < losses[0], losses
+    p = planes.cpu().numpy()
+    assert p.min() >= 0.0 and p.max() <= 1.0
 
 
 def test_fused_l2_loss(ctx):
@@ -341,3 +363,49 @@ def test_fused_step_gradient_clear(ctx, w):
         assert _rel_err(f_gp.cpu().numpy(), gp.cpu().numpy()) <= 1e-5
         assert abs(float(out[3][0]) - float(gs[0])) <= 1e-5 * max(abs(float(gs[0])), 1e-3)
         assert abs(float(out[4][0]) - float(loss[0])) <= 1e-5 * float(loss[0])
+
+
+def test_fused_step_of_a_4096x2160_frame(ctx):
+    """A 4096 x 2160 frame has 8640 blocks of 64 x 16 pixels, more than a
+    quarter of the partial-sum scratch: the fused step still equals the
+    three separate calls (image, src and loss gradient bit for bit, loss
+    within f32 summation order, gradients within 1e-5)."""
+    r, col, _, _ = _setup(ctx, seed=11, size=256)
+    v = View(x=120.3, y=77.7, height=280.0, angle=0.9, horizon=700.0, distance=400.0, fov=1.2,
+             h=2160, w=4096, d=0.001, z0=0.4, n_slices=900)
+    planes = torch.from_numpy(planes_of(col)).cuda()
+    rng = np.random.default_rng(41)
+    target = torch.from_numpy(rng.random((3, v.h, v.w), dtype=np.float32)).cuda()
+    img, src = r.forward_raw(planes, 0.37, v)
+    gimg, loss = r.l2_loss_grad(img, target)
+    gp, gs = r.vjp_raw(planes, src, gimg, v)
+    f_gimg = torch.empty_like(gimg)
+    f_img, f_src, f_gp, f_gs, f_loss = r.l2_step(planes, 0.37, v, target, gimg=f_gimg)
+    torch.cuda.synchronize()
+    assert torch.equal(f_img, img) and torch.equal(f_src, src) and torch.equal(f_gimg, gimg)
+    assert abs(float(f_loss[0]) - float(loss[0])) <= 1e-5 * float(loss[0])
+    assert _rel_err(f_gp.cpu().numpy(), gp.cpu().numpy()) <= 1e-5
+    assert abs(float(f_gs[0]) - float(gs[0])) <= 1e-5 * max(abs(float(gs[0])), 1.0)
+
+
+def test_fused_step_refuses_frames_beyond_its_partials(ctx):
+    """8192 x 2112 needs 16896 loss partials (more than 16384): the step is
+    refused with a message naming the limit, before anything is enqueued
+    (the outputs keep their contents)."""
+    from futspace_amd.renderer import FutspaceError
+    r, col, _, _ = _setup(ctx, seed=11, size=128)
+    v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=60.0, distance=300.0, fov=1.2,
+             h=2112, w=8192, d=0.01, z0=0.5)
+    planes = torch.from_numpy(planes_of(col)).cuda()
+    target = torch.zeros((3, v.h, v.w), dtype=torch.float32, device="cuda")
+    img = torch.full_like(target, 7.0)
+    gimg = torch.full_like(target, 3.0)
+    src = torch.full((v.h, v.w), 5, dtype=torch.int32, device="cuda")
+    gp = torch.full_like(planes, 9.0)
+    gs = torch.full((1,), 4.0, device="cuda")
+    loss = torch.full((1,), 2.0, device="cuda")
+    with pytest.raises(FutspaceError, match="16384"):
+        r.l2_step(planes, 0.37, v, target, img, src, gimg, gp, gs, loss)
+    torch.cuda.synchronize()
+    assert bool((img == 7.0).all()) and bool((gimg == 3.0).all()) and bool((src == 5).all())
+    assert bool((gp == 9.0).all()) and float(gs[0]) == 4.0 and float(loss[0]) == 2.0

@@ -228,3 +228,26 @@ def test_rows_beyond_the_limit_are_rejected(ctx, small_scene):
              h=4097, w=8, d=0.004, z0=0.5)
     with pytest.raises(RuntimeError):
         ctx.render(m, v)
+
+
+def test_tuning_belongs_to_its_context(ctx):
+    """fsx_set_tuning changes the kernel shape of its own context only; each
+    context renders the same frame bit for bit in its shape."""
+    from futspace_amd import Context
+    col, hts = scene.scene_maps(5, 256)
+    v = View(x=120.3, y=17.7, height=240.0, angle=2.2, horizon=90.0, distance=500.0, fov=1.2,
+             sky_color=sky_color(0.1), h=400, w=300, d=0.002, z0=0.5)
+    shaded, alt = O.update_map(col, hts)
+    want = O.render_off(v, shaded, alt)
+    base = ctx.tuning()
+    c2 = Context(device=0)
+    try:
+        assert c2.tuning() == base
+        c2.set_tuning(4, 1, 1024)
+        assert c2.tuning() == (4, 1, 1024)
+        assert ctx.tuning() == base
+        for c in (ctx, c2):
+            _check(c.render(c.map(col, hts), v), want)
+    finally:
+        c2.close()
+    assert ctx.tuning() == base
