@@ -118,6 +118,8 @@ FUTSPACE_SIGNATURES = {
     "fsx_l2_loss": (ctypes.c_int, [P, P, P, I64, I32, P, P]),
     "fsx_render_f32_l2_step": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, P, I32, I32, F32,
                                               P, I32, P, P, P, P, P, P]),
+    "fsx_render_f32_l2_step_noalloc": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, P, I32, I32, F32,
+                                              P, I32, P, P, P, P, P, P]),
     "fsx_render_stateless": (ctypes.c_int, [P, P, P, I32, I32, ctypes.POINTER(FsxView), P]),
     "fsx_render2": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, I64]),
     "fsx_context_device": (ctypes.c_int, [P]),

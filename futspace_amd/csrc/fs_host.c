@@ -1697,30 +1697,43 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
     return 0;
 }
 
-int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
-                           const struct fsx_view *view, const float *dev_planes,
-                           const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
-                           const float *dev_target, int32_t accumulate, float *dev_img,
-                           int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
-                           float *dev_gsky, float *dev_loss)
+/* The fused step behind both entries; `noalloc` takes the partial sums
+ * from the device's fixed scratch array instead of the context's buffer
+ * (which the first use allocates). */
+static int l2_step(struct futhark_context *ctx, const char *what, int noalloc,
+                   const struct fsx_map *map, const struct fsx_view *view,
+                   const float *dev_planes, const float *dev_sf, int32_t hc, int32_t wc,
+                   float sky_scale, const float *dev_target, int32_t accumulate, float *dev_img,
+                   int32_t *dev_src, float *dev_gimg, float *dev_gplanes, float *dev_gsky,
+                   float *dev_loss)
 {
-    REQUIRE_ARGS(ctx, map && view, "fsx_render_f32_l2_step");
+    REQUIRE_ARGS(ctx, map && view, what);
     fsk_frame f;
-    if (diff_frame(ctx, "fsx_render_f32_l2_step", view, hc, wc, &f))
+    if (diff_frame(ctx, what, view, hc, wc, &f))
         return 1;
     if (!dev_planes || !dev_sf || !dev_gplanes || !dev_gsky || !dev_loss ||
         ((!dev_target || !dev_img || !dev_src || !dev_gimg) && view->h > 0 && view->w > 0)) {
-        set_error(ctx, "fsx_render_f32_l2_step: null buffer");
+        set_error(ctx, "%s: null buffer", what);
         return 1;
     }
     const size_t nplanes = (size_t)3 * (size_t)hc * (size_t)wc;
     /* the loss and sky partials take half of the scratch each */
     if (step_blocks(view->w, view->h) > FS_PARTIALS / 2) {
-        set_error(ctx, "fsx_render_f32_l2_step: a %d x %d frame has %lld blocks of %d x %d "
-                  "pixels, more than the %d loss partial sums of the fused step", view->w,
-                  view->h, (long long)step_blocks(view->w, view->h), FS_STEP_BLOCK_COLS,
+        set_error(ctx, "%s: a %d x %d frame has %lld blocks of %d x %d pixels, more than the "
+                  "%d loss partial sums of the fused step", what, view->w, view->h,
+                  (long long)step_blocks(view->w, view->h), FS_STEP_BLOCK_COLS,
                   FS_STEP_BLOCK_ROWS, FS_PARTIALS / 2);
         return 1;
+    }
+    float *part = NULL;
+    if (noalloc) {
+        int64_t cap = 0;
+        part = fsk_scratch(ctx->device, &cap);
+        if (!part || cap < FS_PARTIALS) {
+            set_error(ctx, "%s: the device scratch of %lld floats is not set up", what,
+                      (long long)cap);
+            return 1;
+        }
     }
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     int pe = prof_begin(ctx);
@@ -1733,8 +1746,7 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
         prof_end(ctx, pe);
         return 0;
     }
-    float *part;
-    if (partials(ctx, &part))
+    if (!noalloc && partials(ctx, &part))
         return 1;
     fsk_map m;
     m.hq = map->t->d_hq;
@@ -1766,6 +1778,30 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
                                    accumulate != 0, part + q, ctx->stream));
     prof_end(ctx, pe);
     return 0;
+}
+
+int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
+                           const struct fsx_view *view, const float *dev_planes,
+                           const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
+                           const float *dev_target, int32_t accumulate, float *dev_img,
+                           int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
+                           float *dev_gsky, float *dev_loss)
+{
+    return l2_step(ctx, "fsx_render_f32_l2_step", 0, map, view, dev_planes, dev_sf, hc, wc,
+                   sky_scale, dev_target, accumulate, dev_img, dev_src, dev_gimg, dev_gplanes,
+                   dev_gsky, dev_loss);
+}
+
+int fsx_render_f32_l2_step_noalloc(struct futhark_context *ctx, const struct fsx_map *map,
+                                   const struct fsx_view *view, const float *dev_planes,
+                                   const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
+                                   const float *dev_target, int32_t accumulate, float *dev_img,
+                                   int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
+                                   float *dev_gsky, float *dev_loss)
+{
+    return l2_step(ctx, "fsx_render_f32_l2_step_noalloc", 1, map, view, dev_planes, dev_sf, hc,
+                   wc, sky_scale, dev_target, accumulate, dev_img, dev_src, dev_gimg,
+                   dev_gplanes, dev_gsky, dev_loss);
 }
 
 int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *dev_target,

@@ -49,6 +49,11 @@ int32_t fsk_frame_tame(const fsk_frame *f);
  * later calls on the same device return at once. */
 int fsk_device_init(void);
 
+/* The device address of `device`'s fixed scratch array (set up by
+ * fsk_device_init; NULL before) and its size in floats.  Shared by every
+ * caller on that device. */
+float *fsk_scratch(int32_t device, int64_t *n_floats);
+
 /* render2 (ray march) of one frame into out[rows][pitch]: d_cs/d_sn[cols]
  * the host's cos / sin of angle + horz_ang per column, d_color the map's
  * unshaded colours [hrows][hcols], n = i32(distance / 4). */

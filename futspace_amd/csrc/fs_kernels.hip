@@ -385,6 +385,13 @@ __device__ __forceinline__ uint32_t mix_channel_exact(float m1, float s1, float 
  * built on the host with the C library's correctly rounded sqrtf and is
  * checked against the sqrt sequence for every f32 in [0, 1] by the device
  * self-test. */
+/* Scratch of the allocation-free fused step (fsx_render_f32_l2_step_noalloc):
+ * per-workgroup partial sums, kScratchFloats floats per device, held until
+ * the process ends.  Host code reaches it only through the address
+ * hipGetSymbolAddress returns for the current device (fsk_scratch). */
+constexpr int kScratchFloats = 32768;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
 constexpr uint32_t kQLo = 0x37810000u;           /* below sqrt(t) * 255 >= 1 */
 constexpr int kQBuckets = 2048;                  /* (bits(1.0f) - kQLo) >> 16 = 2047 */
 __device__ uint32_t g_qtab[kQBuckets];
@@ -2109,6 +2116,7 @@ void qtab_build(uint32_t *qt)
 
 std::mutex g_init_mu;
 bool g_init_done[64];
+float *g_scratch_ptr[64];    /* device address of `scratch` on each device */
 
 }  // namespace
 
@@ -2134,9 +2142,26 @@ int fsk_device_init(void)
                               hipMemcpyHostToDevice);
         if (e != hipSuccess)
             return e;
+        /* this device's copy of the scratch array: its address, not the
+         * array's host-side name, is what kernels and copies may use */
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[dev] = (float *)p;
         g_init_done[dev] = true;
     }
     return hipSuccess;
+}
+
+float *fsk_scratch(int32_t device, int64_t *n_floats)
+{
+    if (n_floats)
+        *n_floats = kScratchFloats;
+    if (device < 0 || device >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(g_init_mu);
+    return g_init_done[device] ? g_scratch_ptr[device] : nullptr;
 }
 
 int32_t fsk_tile_columns(int32_t max_rows, const fsk_tuning *t)

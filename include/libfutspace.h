@@ -268,6 +268,24 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
                            int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
                            float *dev_gsky, float *dev_loss);
 
+/* fsx_render_f32_l2_step that allocates nothing: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving or training loops).  Same
+ * arguments, results and frame limit (16384 blocks of 64 x 16 pixels);
+ * the per-workgroup partial sums live in a fixed scratch array of 32768
+ * floats (128 KiB) per device, set up when the first context on the
+ * device is created and held until the process ends, instead of the
+ * context buffer fsx_render_f32_l2_step allocates on first use.  Every
+ * call on a device shares that device's array, so calls that overlap on
+ * one GPU (from several contexts or streams) race; calls ordered on one
+ * stream do not. */
+int fsx_render_f32_l2_step_noalloc(struct futhark_context *ctx, const struct fsx_map *map,
+                                   const struct fsx_view *view, const float *dev_planes,
+                                   const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
+                                   const float *dev_target, int32_t accumulate, float *dev_img,
+                                   int32_t *dev_src, float *dev_gimg, float *dev_gplanes,
+                                   float *dev_gsky, float *dev_loss);
+
 /* Render kernel shape, as chosen by the autotuner: columns per workgroup
  * tile (4 or 8), workgroups per CU (1 of 1024 threads, or 2 of 512 whose
  * depth chunks are shortened until two tiles fit in LDS) and depth slices
