@@ -507,7 +507,7 @@ __device__ __forceinline__ uint32_t color_of_quad(uint4 q, const float *sq, floa
 }
 
 constexpr int kShortSpan = 40;   /* span rows a lane paints itself */
-constexpr int kGather = 2;       /* height gathers a lane keeps in flight */
+constexpr int kGather = 4;       /* height gathers a lane keeps in flight */
 constexpr int kColourBatch = 1;  /* visible samples a lane colours per pass */
 
 /* Dynamic LDS of a tile render of tc columns by nt threads: slice table
@@ -668,30 +668,41 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         const int32_t cn = min(csz, n - base);
         /* phase 0: the chunk's slice records in slot order (slot j*SG + g
          * holds slice g*kr + j; slots past the chunk repeat its last
-         * slice).  Computing them per tile is cheaper than copying a
-         * precomputed per-view table, even in batches. */
-        const int32_t nslots = ((kr + GATHER - 1) / GATHER) * GATHER * SG;
-        if (f.tame) {
-            const float rfw = 1.0f / f.fw;
-            for (int32_t o = tid; o < nslots; o += NT) {
-                const int32_t s = min((o % SG) * kr + o / SG, cn - 1);
-                Slice sl = slice_params_tame(f, base + s, rfw);
-                tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
-                tz[o] = sl.iz;
-            }
-        } else {
-            for (int32_t o = tid; o < nslots; o += NT) {
-                const int32_t s = min((o % SG) * kr + o / SG, cn - 1);
-                Slice sl = slice_params(f, base + s);
-                tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
-                tz[o] = sl.iz;
-            }
+         * slice).  Each wave computes the slots of its own slice groups --
+         * the only ones its projection reads -- so no workgroup barrier
+         * separates this phase from the next: a wave starts projecting as
+         * soon as its own records are in LDS.  Computing them per tile is
+         * cheaper than copying a precomputed per-view table, even in
+         * batches. */
+        {
+            constexpr int GPW = 64 / TC;             /* slice groups per wave */
+            const int32_t nj = ((kr + GATHER - 1) / GATHER) * GATHER;
+            auto table = [&](auto tame_tag) {
+                constexpr bool TAMEP = decltype(tame_tag)::value;
+                const float rfw = TAMEP ? 1.0f / f.fw : 0.0f;
+                for (int32_t q = lane; q < nj * GPW; q += 64) {
+                    const int32_t gq = wave * GPW + q % GPW, j = q / GPW;
+                    const int32_t o = j * SG + gq;
+                    const int32_t s = min(gq * kr + j, cn - 1);
+                    Slice sl = TAMEP ? slice_params_tame(f, base + s, rfw)
+                                     : slice_params(f, base + s);
+                    tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
+                    tz[o] = sl.iz;
+                }
+            };
+            if (f.tame)
+                table(std::true_type{});
+            else
+                table(std::false_type{});
         }
         if (tid == 0)
             sh.qn = 0u;
-        if (base == 0)
-            tables.store(sh, tid);
-        __syncthreads();
+        /* the wave reads back what its own lanes wrote: LDS operations of a
+         * wave complete in order; the fence keeps the compiler from moving
+         * the reads above the writes */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
         /* phase 1: rows are < 2^16, two per register.  Branch-free in
          * groups of GATHER samples: all of a group's height gathers are
@@ -784,8 +795,11 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         if (lane >= 64 - TC)
             sh.wtot[wave][c] = v;
         /* the chunk's carry-in is read before the barrier: the last slice
-         * group overwrites it in phase 3 without a barrier in between */
-        const int32_t carry_in = sh.carry[c];
+         * group overwrites it in phase 3 without a barrier in between (the
+         * first chunk's is H) */
+        const int32_t carry_in = base == 0 ? H : sh.carry[c];
+        if (base == 0)
+            tables.store(sh, tid);
         __syncthreads();
         /* exclusive prefix over the waves: each lane reads the totals of
          * the waves before its own (same address across the wave's lanes
@@ -909,6 +923,10 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         if (done)
             break;
     }
+
+    /* no chunk ran (no slices): publish the initial y-buffer (H, all sky) */
+    if (n <= 0)
+        __syncthreads();
 
     /* long spans, painted by the whole workgroup */
     const uint32_t nl = sh.nlong;
