@@ -26,6 +26,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--settle-s", type=float, default=0.5,
+                    help="untimed rendering before the warmup steps, while GPU clocks ramp up")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work for the baseline sample")
@@ -347,6 +349,14 @@ def main():
     out = torch.empty((H, W), dtype=torch.int32, device="cuda")
 
     with torch.cuda.stream(stream):
+        # settle: a fresh box's GPU clocks ramp up under load for a few
+        # hundred milliseconds; render untimed until that has passed, so
+        # that the timed steps measure the steady state
+        t_settle = time.perf_counter()
+        while time.perf_counter() - t_settle < args.settle_s:
+            for v in views:
+                ctx.render(m, v, out)
+            torch.cuda.synchronize()
         for i in range(args.warmup):
             ctx.render(m, views[i % len(views)], out)
         torch.cuda.synchronize()

@@ -841,6 +841,7 @@ static void frame_of_view(const struct fsx_view *v, fsk_frame *f)
     f->z0 = v->z0;
     f->scale = v->scale > 0.0f ? v->scale : (float)(v->w / 2);
     f->fw = (float)v->w;
+    f->rfw = 1.0f / f->fw;
     f->n = v->n_slices > 0 ? v->n_slices : zs_count(v->distance, v->d, v->z0);
     f->rows = v->h;
     f->cols = v->w;

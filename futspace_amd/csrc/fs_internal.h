@@ -33,6 +33,7 @@ typedef struct {
     float cam_x, cam_y, cam_h, horizon;
     float Lx, Ly, Rx, Ry;
     float d, z0, scale, fw;
+    float rfw;           /* 1 / fw, correctly rounded (the host's f32 division) */
     int32_t n, rows, cols;
     uint32_t sky;
     int32_t tame;        /* fsk_frame_tame(this); set by the launchers */
@@ -105,7 +106,7 @@ int32_t fsk_batch_grid(int32_t max_rows, const fsk_tuning *t);
 /* Reset the claim counters and run the persistent batched render. */
 int fsk_batch_run(const fsk_batch *b, hipStream_t s);
 
-/* Device self-test `which` (0-6) of the arithmetic shortcuts; adds the
+/* Device self-test `which` (0-7) of the arithmetic shortcuts; adds the
  * number of mismatching inputs to *d_bad. */
 int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s);
 

@@ -132,6 +132,14 @@ __device__ __forceinline__ float sqrt_unit(float t)
     return s;
 }
 
+/* Correctly rounded sqrt of t >= 0 (or NaN): sqrt_unit's neighbour
+ * correction holds for every non-negative f32; device self-test 7 checks
+ * it against the correctly rounded sqrt on all of them. */
+__device__ __forceinline__ float sqrt_rn(float t)
+{
+    return sqrt_unit(t);
+}
+
 __device__ __forceinline__ uint32_t from_rgba(float r, float g, float b, float a)
 {
     return (chan_u8(a) << 24) | (chan_u8(r) << 16) | (chan_u8(g) << 8) | chan_u8(b);
@@ -256,9 +264,10 @@ __device__ __forceinline__ float div_rn(float a, float b, float rb)
 
 /* slice_params of a tame frame (z >= z0 > 0, finite scalars) with the two
  * divisions by the frame width and the reciprocal of z taken by rcp_rn and
- * div_rn; rfw = RN(1 / fw).  Bit-identical to slice_params there. */
-__device__ __forceinline__ Slice slice_params_tame(const fsk_frame &f, int32_t k, float rfw)
+ * div_rn (f.rfw = RN(1 / fw)).  Bit-identical to slice_params there. */
+__device__ __forceinline__ Slice slice_params_tame(const fsk_frame &f, int32_t k)
 {
+    const float rfw = f.rfw;
     float i = (float)(k + 1);
     float z = (i / 2.0f) * ((2.0f * f.z0) + ((i - 1.0f) * f.d));
     float left_x = f.Lx * z, left_y = f.Ly * z;
@@ -679,13 +688,11 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
             const int32_t nj = ((kr + GATHER - 1) / GATHER) * GATHER;
             auto table = [&](auto tame_tag) {
                 constexpr bool TAMEP = decltype(tame_tag)::value;
-                const float rfw = TAMEP ? 1.0f / f.fw : 0.0f;
                 for (int32_t q = lane; q < nj * GPW; q += 64) {
                     const int32_t gq = wave * GPW + q % GPW, j = q / GPW;
                     const int32_t o = j * SG + gq;
                     const int32_t s = min(gq * kr + j, cn - 1);
-                    Slice sl = TAMEP ? slice_params_tame(f, base + s, rfw)
-                                     : slice_params(f, base + s);
+                    Slice sl = TAMEP ? slice_params_tame(f, base + s) : slice_params(f, base + s);
                     tl[o] = make_float4(sl.sx, sl.sy, sl.dx, sl.dy);
                     tz[o] = sl.iz;
                 }
@@ -1224,7 +1231,9 @@ __global__ void k_shade(const uint16_t *__restrict__ hp, int32_t hrows, int32_t 
  * 5: rcp_rn(z) == 1 / z (correctly rounded) for every f32 z in
  *    [2^-126, 2^126).
  * 6: div_rn(a, w, RN(1 / w)) == a / w for every finite f32 a and every
- *    frame width w of kSelftestWidths. */
+ *    frame width w of kSelftestWidths.
+ * 7: sqrt_rn(t) == correctly rounded sqrt(t) for every f32 t >= 0
+ *    (+inf included) and NaN stays NaN. */
 constexpr int32_t kSelftestWidths[] = {1920, 1280, 800, 640, 320, 3840, 128, 96, 1024,
                                        2048, 4096, 240, 1, 3, 7, 1000, 1366};
 constexpr int kNSelftestWidths = sizeof kSelftestWidths / sizeof kSelftestWidths[0];
@@ -1233,6 +1242,7 @@ __global__ void k_selftest(int which, unsigned long long *bad)
     uint64_t n = which == 0 ? 256ull
                  : (which == 1 || which == 4) ? 0x3F800001ull
                  : which == 5 ? 0x7E000000ull
+                 : which == 7 ? 0x7F800001ull
                  : which == 6 ? (uint64_t)kNSelftestWidths << 32
                               : 0x100000000ull;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1240,7 +1250,12 @@ __global__ void k_selftest(int which, unsigned long long *bad)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         uint32_t u = (uint32_t)i;
         bool ok = true;
-        if (which == 5) {
+        if (which == 7) {
+            const float t = __uint_as_float(u);
+            ok = __float_as_uint(sqrt_rn(t)) == __float_as_uint(__builtin_sqrtf(t));
+            const float nan = __uint_as_float(0x7FC00000u | (u & 0x3FFFFFu));
+            ok = ok && (sqrt_rn(nan) != sqrt_rn(nan));
+        } else if (which == 5) {
             u += 0x00800000u;
             const float z = __uint_as_float(u);
             ok = __float_as_uint(rcp_rn(z)) == __float_as_uint(1.0f / z);
@@ -1622,11 +1637,18 @@ struct DiffArgs {
     float *sky_part;           /* the fused forward's per-workgroup sky terms */
 };
 
+/* The relaxed argb.mix channel: sqrt(m1' a^2 + m2' b^2) with the weights
+ * normalised by m1 + m2 -- a division that is exact and skipped when the
+ * sum is exactly 1 (bilinear weights at |x| >= 1). */
 __device__ __forceinline__ float mixf(float m1, float a, float m2, float b)
 {
     float m12 = m1 + m2;
-    float m1n = m1 / m12, m2n = m2 / m12;
-    return __builtin_sqrtf((m1n * (a * a)) + (m2n * (b * b)));
+    float m1n = m1, m2n = m2;
+    if (m12 != 1.0f) {
+        m1n = m1 / m12;
+        m2n = m2 / m12;
+    }
+    return sqrt_rn((m1n * (a * a)) + (m2n * (b * b)));
 }
 
 /* One visible sample: texel offsets (fy,fx) (fy,cx) (cy,fx) (cy,cx) and
@@ -1644,7 +1666,7 @@ template <bool P2>
 __device__ __forceinline__ Bilin bilin_of(const fsk_frame &f, int32_t k, int32_t col, int32_t hc,
                                           int32_t wc)
 {
-    Slice sl = slice_params(f, k);
+    Slice sl = f.tame ? slice_params_tame(f, k) : slice_params(f, k);
     float fcol = (float)col;
     float x = sl.sx + (fcol * sl.dx);
     float y = sl.sy + (fcol * sl.dy);
@@ -1876,8 +1898,15 @@ __device__ __forceinline__ void sample_terms(const DiffArgs &a, int32_t k, int32
     float sv[3][4], px[3];
     sample_colour(a, b, plane, sv, px);
     const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
-    const float wx0n = b.wx0 / wxs, wx1n = b.wx1 / wxs;
-    const float wy0n = b.wy0 / wys, wy1n = b.wy1 / wys;
+    float wx0n = b.wx0, wx1n = b.wx1, wy0n = b.wy0, wy1n = b.wy1;
+    if (wxs != 1.0f) {            /* exact and skipped when the sum is 1 */
+        wx0n = b.wx0 / wxs;
+        wx1n = b.wx1 / wxs;
+    }
+    if (wys != 1.0f) {
+        wy0n = b.wy0 / wys;
+        wy1n = b.wy1 / wys;
+    }
     const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
     float q[3];
 #pragma unroll
@@ -2529,7 +2558,7 @@ int32_t fsk_render_max_rows(void)
 
 int fsk_selftest(int which, unsigned long long *d_bad, hipStream_t s)
 {
-    if (which < 0 || which > 6)
+    if (which < 0 || which > 7)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_selftest, dim3(4096), dim3(256), 0, s, which, d_bad);
     return hipGetLastError();

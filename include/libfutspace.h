@@ -321,7 +321,8 @@ uint32_t fsx_sky(float sun_height);
 /* Run device self-test `which` of the kernels' arithmetic shortcuts
  * (0: byte/255, 1: sqrt on [0,1], 2: channel quantisation, 3: screen-row
  * clamp, 4: the table quantisation of mixed channels, 5: the slice
- * table's reciprocal, 6: its divisions by the frame width) and report how many inputs disagree with the plain formula and
+ * table's reciprocal, 6: its divisions by the frame width, 7: sqrt of the
+ * relaxed colour mix) and report how many inputs disagree with the plain formula and
  * the smallest such input's bit pattern (-1 when none; may be NULL). */
 int fsx_selftest(struct futhark_context *ctx, int which, int64_t *mismatches,
                  int64_t *first_bad);

@@ -153,7 +153,7 @@ def test_render_into_torch_tensor(ctx, small_scene):
     assert not got[:, 128:].any(), "render wrote past the frame width"
 
 
-@pytest.mark.parametrize("which", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("which", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_device_arithmetic_selftest(ctx, which):
     """The kernels' shortcuts (byte/255 by reciprocal + FMA correction, sqrt
     on [0,1] without scaling, median-based quantisation, row clamp, the
