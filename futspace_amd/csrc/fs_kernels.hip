@@ -614,6 +614,18 @@ __device__ __forceinline__ uint32_t tile_value(uint32_t v, uint32_t sky)
     return v ? v - 1u : 0xFFFFFFFFu;
 }
 
+/* tile_value of a cell that is either painted with a non-zero value (v) or,
+ * above its column's final y-buffer, unpainted (sky). */
+template <int MODE>
+__device__ __forceinline__ uint32_t painted_value(uint32_t v, bool sky_cell, uint32_t sky)
+{
+    if (MODE == kOutColour)
+        return sky_cell ? sky : v;
+    if (MODE == kOutNearest)
+        return sky_cell ? sky : v ^ sky;
+    return sky_cell ? 0xFFFFFFFFu : v - 1u;
+}
+
 /* Texel index of the stateless program's sample at (x, y): i32.f32 of
  * each coordinate (x86 truncation), floor-mod by the map size. */
 template <bool P2>
@@ -935,13 +947,13 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     if (n <= 0)
         __syncthreads();
 
-    /* long spans, painted by the whole workgroup */
+    /* long spans: each wave paints every NW-th one with its 64 lanes */
     const uint32_t nl = sh.nlong;
-    for (uint32_t li = 0; li < nl; li++) {
+    for (uint32_t li = (uint32_t)wave; li < nl; li += NW) {
         uint32_t d = longs[2 * li], colour = longs[2 * li + 1];
         int32_t cc = (int32_t)(d >> 26), r0 = (int32_t)((d >> 13) & 0x1FFFu);
         int32_t r1 = (int32_t)(d & 0x1FFFu);
-        for (int32_t r = r0 + tid; r < r1; r += NT)
+        for (int32_t r = r0 + lane; r < r1; r += 64)
             stripe[cc * Hp + r] = colour;
     }
     if (nl)
@@ -967,7 +979,7 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
             uint32_t px[4];
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                px[j] = tile_value<MODE>(cell(part * 4 + j, r, yb[j]), f.sky);
+                px[j] = painted_value<MODE>(stripe[(part * 4 + j) * Hp + r], r < yb[j], f.sky);
             uint32_t *dst = out + (size_t)r * (size_t)pitch + col0 + part * 4;
             if (vec) {
                 *(uint4 *)dst = make_uint4(px[0], px[1], px[2], px[3]);
