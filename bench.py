@@ -45,20 +45,41 @@ def parse():
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_render_plain_2048_1080p.md")
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "render_counters.md")
+# non-packed VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+
+
+def _profile_value(path, key):
+    """A `key = value` line of a committed rocprofv3 summary; None if absent."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key):
+                    return float(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic():
     """HBM bytes per k_render launch on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the
     counter reads for a 64 MiB copy); None if the summary is absent."""
-    try:
-        with open(TRAFFIC_PROFILE) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
-                    return float(line.split("=", 1)[1])
-    except OSError:
-        pass
-    return None
+    return _profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch")
+
+
+def valu_roofline(kern_ms):
+    """The VALU side of the kernel: SQ_INSTS_VALU per launch (committed
+    rocprofv3 pass) x 64 lanes against the non-packed lane-op peak."""
+    insts = _profile_value(COUNTER_PROFILE, "valu_insts_per_launch")
+    if insts is None:
+        return None
+    achieved = insts * 64 / (kern_ms * 1e-3)
+    return {"insts_per_launch": insts, "lane_ops_per_launch": insts * 64,
+            "achieved_lane_ops_per_s": achieved, "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+            "frac": achieved / VALU_PEAK_LANE_OPS,
+            "source": os.path.relpath(COUNTER_PROFILE, ROOT)}
 
 
 def calib_copy():
@@ -423,7 +444,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
-                     "kernel": "k_render", "algorithmic_bytes_per_launch": bytes_per_frame},
+                     "kernel": "k_render", "algorithmic_bytes_per_launch": bytes_per_frame,
+                     "valu": valu_roofline(kern_ms)},
     }
     if not args.no_batch:
         ctx.set_stream(None)
