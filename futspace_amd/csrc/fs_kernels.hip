@@ -126,8 +126,11 @@ __device__ __forceinline__ float sqrt_unit(float t)
     s = (rp > 0.0f) ? sp : s;
     bool tiny = (t < 0x1p-96f) && (t > 0.0f);
     if (__ballot(tiny)) {
+        /* the empty asm keeps the general sqrt inside the branch */
+        float tt = t;
+        asm volatile("" : "+v"(tt));
         if (tiny)
-            s = __builtin_sqrtf(t);
+            s = __builtin_sqrtf(tt);
     }
     return s;
 }
@@ -251,7 +254,8 @@ __device__ __forceinline__ float rcp_rn(float z)
  * a - q * b is exact in one FMA, and q + residual * rb rounds to the
  * nearest quotient).  The residual is not exact for quotients near the
  * subnormal range, and the signs of zero differ, so quotients below 2^-100
- * in magnitude take the division itself.  Device self-test 6 compares it
+ * in magnitude take the division itself, on a wave-uniform branch (a plain
+ * select would have every lane compute it).  Device self-test 6 compares it
  * with the correctly rounded division for every finite f32 a at a set of
  * frame widths b. */
 __device__ __forceinline__ float div_rn(float a, float b, float rb)
@@ -259,7 +263,16 @@ __device__ __forceinline__ float div_rn(float a, float b, float rb)
     float q = a * rb;
     float r = __builtin_fmaf(-q, b, a);
     q = __builtin_fmaf(r, rb, q);
-    return fabsf(q) >= 0x1p-100f ? q : a / b;
+    const bool small = !(fabsf(q) >= 0x1p-100f);
+    if (__ballot(small)) {
+        /* the empty asm keeps the compiler from hoisting the division out
+         * of the branch (it may otherwise evaluate it on every wave) */
+        float aa = a;
+        asm volatile("" : "+v"(aa));
+        if (small)
+            q = aa / b;
+    }
+    return q;
 }
 
 /* slice_params of a tame frame (z >= z0 > 0, finite scalars) with the two
