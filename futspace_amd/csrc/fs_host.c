@@ -61,8 +61,8 @@ struct futhark_context {
 };
 
 #define FS_PARTIALS 32768     /* floats in d_partials */
-#define FS_STEP_BLOCK_COLS 64 /* pixel blocks of the colour / loss pass and */
-#define FS_STEP_BLOCK_ROWS 16 /* of the adjoint: one partial sum per block */
+#define FS_STEP_BLOCK_COLS 64 /* pixel blocks of the adjoint (and of the   */
+#define FS_STEP_BLOCK_ROWS 16 /* fused step): one partial sum per block    */
 
 /* Pixel blocks of a w x h frame in the differentiable passes. */
 static int64_t step_blocks(int32_t w, int32_t h)
@@ -1713,7 +1713,7 @@ static int l2_step(struct futhark_context *ctx, const char *what, int noalloc,
     if (diff_frame(ctx, what, view, hc, wc, &f))
         return 1;
     if (!dev_planes || !dev_sf || !dev_gplanes || !dev_gsky || !dev_loss ||
-        ((!dev_target || !dev_img || !dev_src || !dev_gimg) && view->h > 0 && view->w > 0)) {
+        ((!dev_target || !dev_src) && view->h > 0 && view->w > 0)) {
         set_error(ctx, "%s: null buffer", what);
         return 1;
     }
@@ -1758,25 +1758,20 @@ static int l2_step(struct futhark_context *ctx, const char *what, int noalloc,
     m.crows = hc;
     m.ccols = wc;
     /* unless accumulating, the gradient planes are cleared by the slice
-     * render's spare workgroups (CUs its tiles leave idle) or else inside
-     * the colour launch */
+     * render's spare workgroups (CUs its tiles leave idle) or else by a
+     * launch of their own */
     int32_t cleared = 0;
     HIPCHECK(ctx, fsk_render_slices_clear(&m, &f, &ctx->tune, dev_src, view->w,
                                           accumulate ? NULL : dev_gplanes, nplanes, &cleared,
                                           ctx->stream));
-    /* the forward's loss and sky partials take the scratch's two halves
-     * and are summed by the adjoint's first workgroup (the sky term of the
-     * loss gradient is known once g is); given them, the adjoint writes no
-     * partials of its own */
+    if (!accumulate && !cleared)
+        HIPCHECK(ctx, fsk_zero_f32(dev_gplanes, nplanes, ctx->stream));
+    /* colour, loss terms and adjoint in one launch; the loss and sky
+     * partials take the scratch's two halves */
     const int32_t q = FS_PARTIALS / 2;
-    int32_t nloss = 0;
-    HIPCHECK(ctx, fsk_diff_color_l2(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_target,
-                                    dev_img, dev_gimg, dev_loss, accumulate != 0,
-                                    accumulate || cleared ? NULL : dev_gplanes, nplanes, part, q,
-                                    &nloss, part + q, ctx->stream));
-    HIPCHECK(ctx, fsk_diff_grad_l2(&f, dev_planes, dev_sf, hc, wc, dev_src, dev_gimg, dev_gplanes,
-                                   dev_gsky, accumulate != 0, NULL, 0, part, nloss, dev_loss,
-                                   accumulate != 0, part + q, ctx->stream));
+    HIPCHECK(ctx, fsk_diff_step_l2(&f, dev_planes, dev_sf, hc, wc, sky_scale, dev_src, dev_target,
+                                   dev_img, dev_gimg, dev_gplanes, dev_gsky, accumulate != 0,
+                                   dev_loss, accumulate != 0, part, part + q, q, ctx->stream));
     prof_end(ctx, pe);
     return 0;
 }

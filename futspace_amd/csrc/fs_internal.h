@@ -144,27 +144,17 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
                   int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
                   float *d_gsky, int32_t sky_accumulate, float *d_partials, int32_t n_partials,
                   hipStream_t s);
-/* Forward colour + L2 loss terms of the fused step: img, g = 2 (img - tgt)
- * and *d_loss (= or += sum (img - tgt)^2); clears d_zero[zero_n] (if not
- * null) in the same launch.  With n_loss_partials non-null the loss
- * partials are left in d_partials (their count stored there) for
- * fsk_diff_grad_l2 to sum; d_sky_part (if not null) receives the
- * per-workgroup sky terms of the loss gradient. */
-int fsk_diff_color_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
-                      int32_t wc, float sky_scale, const int32_t *d_src, const float *d_tgt,
-                      float *d_img, float *d_g, float *d_loss, int32_t loss_accumulate,
-                      float *d_zero, size_t zero_n, float *d_partials, int32_t n_partials,
-                      int32_t *n_loss_partials, float *d_sky_part, hipStream_t s);
-/* fsk_diff_grad that also sums the loss and sky partials d_loss_part and
- * d_sky_part [loss_n] of the preceding fsk_diff_color_l2 (left unsummed
- * through its n_loss_partials, with d_sky_part given) into *d_loss and
- * *d_gsky; its own sky term is then not computed again.  d_partials must
- * not overlap them. */
-int fsk_diff_grad_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
-                     int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
-                     float *d_gsky, int32_t sky_accumulate, float *d_partials,
-                     int32_t n_partials, const float *d_loss_part, int32_t loss_n, float *d_loss,
-                     int32_t loss_accumulate, const float *d_sky_part, hipStream_t s);
+/* The fused L2 step after the slice render: for each pixel the colour px
+ * (written to d_img if not null), g = 2 (px - d_tgt) (written to d_g if not
+ * null), the loss sum (px - tgt)^2 into *d_loss and dL/dcp into d_gcp (which
+ * accumulates; the caller clears it first), dL/dsky_scale into *d_gsky.  The
+ * loss and sky terms go through per-workgroup partials d_loss_part and
+ * d_sky_part [n_partials each], summed in a fixed order. */
+int fsk_diff_step_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                     int32_t wc, float sky_scale, const int32_t *d_src, const float *d_tgt,
+                     float *d_img, float *d_g, float *d_gcp, float *d_gsky,
+                     int32_t sky_accumulate, float *d_loss, int32_t loss_accumulate,
+                     float *d_loss_part, float *d_sky_part, int32_t n_partials, hipStream_t s);
 int fsk_zero_f32(float *d_p, size_t n, hipStream_t s);
 int fsk_fill_u32(uint32_t *d_p, size_t n, uint32_t v, hipStream_t s);
 /* Reductions write one partial sum per workgroup into d_partials (at most

@@ -1655,11 +1655,9 @@ struct DiffArgs {
     float *gcp;            /* backward: dL/dcp accumulated [3][hc][wc] */
     float *gsky;           /* backward: dL/dsky_scale accumulated */
     float *partials;       /* per-workgroup partial sums (loss or sky) */
-    const float *loss_part;    /* backward of the fused step: the forward's */
-    int32_t loss_n;            /* loss and sky partials, summed by */
-    float *loss;               /* workgroup (0, 0) */
-    int32_t loss_accumulate;
-    float *sky_part;           /* the fused forward's per-workgroup sky terms */
+    const float *tgt;      /* fused step: the target frame [3][rows][cols] */
+    float *gout;           /* fused step: dL/dimg written out (may be null) */
+    float *sky_part;       /* fused step: per-workgroup sky terms */
 };
 
 /* The relaxed argb.mix channel: sqrt(m1' a^2 + m2' b^2) with the weights
@@ -1762,80 +1760,6 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
         a.img[ch * fplane + p] = px[ch];
 }
 
-/* First kernel of the fused L2 step: the forward colour of a 64 x 16
- * pixel block (as k_diff_color), the loss terms g = 2 (img - target) and
- * one partial sum of (img - target)^2 per block, summed over the block in
- * a fixed order.  nzero workgroups, spread evenly through the grid (every
- * `every`-th, from the first), clear the gradient planes with float4
- * stores meanwhile, which overlaps the clear with the gather-bound colour
- * work. */
-constexpr int kStepRows = 16;
-template <bool P2>
-__global__ __launch_bounds__(1024) void k_diff_color_l2(DiffArgs a, const float *__restrict__ tgt,
-                                                        float *__restrict__ g, int32_t nbx,
-                                                        int32_t nzero, int32_t every,
-                                                        float4 *__restrict__ zero, size_t zero_n4)
-{
-    int32_t b = blockIdx.x;
-    const int32_t zi = b / every;
-    if (zi < nzero && b % every == 0) {
-        const size_t stride = (size_t)nzero * blockDim.x;
-        for (size_t i = (size_t)zi * blockDim.x + threadIdx.x; i < zero_n4; i += stride)
-            zero[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
-    }
-    b -= min(zi + 1, nzero);    /* the pixel block: zero blocks before it skipped */
-    const int32_t col = (b % nbx) * 64 + (threadIdx.x & 63);
-    const int32_t row = (b / nbx) * kStepRows + (threadIdx.x >> 6);
-    const int32_t H = a.f.rows, W = a.f.cols;
-    float acc = 0.0f, sky = 0.0f;
-    if (col < W && row < H) {
-        const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
-        const size_t p = (size_t)row * W + col;
-        const int32_t k = a.src[p];
-        float px[3];
-        if (k < 0) {
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++)
-                px[ch] = sky_base(ch) * a.sky_scale;
-        } else {
-            float sv[3][4];
-            sample_colour(a, bilin_of<P2>(a.f, k, col, a.hc, a.wc), plane, sv, px);
-        }
-#pragma unroll
-        for (int ch = 0; ch < 3; ch++) {
-            const float d = px[ch] - tgt[ch * fplane + p];
-            a.img[ch * fplane + p] = px[ch];
-            g[ch * fplane + p] = 2.0f * d;
-            acc += d * d;
-            /* d loss / d sky_scale of a sky pixel: g * (0x90, 0x90, 0xE0) / 255 */
-            if (k < 0)
-                sky += (2.0f * d) * sky_base(ch);
-        }
-    }
-    __shared__ float wpart[16], wsky[16];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        acc += __shfl_xor(acc, off);
-        sky += __shfl_xor(sky, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        wpart[threadIdx.x >> 6] = acc;
-        wsky[threadIdx.x >> 6] = sky;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.0f, ts = 0.0f;
-        for (int w = 0; w < 16; w++) {
-            t += wpart[w];
-            ts += wsky[w];
-        }
-        a.partials[b] = t;
-        if (a.sky_part)
-            a.sky_part[b] = ts;
-    }
-}
-
 /* Segmented suffix sum over the lanes of a wave: lanes holding the same
  * key in a contiguous run form a segment, and each segment's first lane
  * ends up with the segment's total of v[0..2].  Returns whether this
@@ -1914,14 +1838,25 @@ __device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uin
  * val[t][ch] = dL/d plane_ch at that texel.  For px = mix(wy0, X1, wy1, X2),
  * X1 = mix(wx0, s0, wx1, s1): d px / d s0 = wy0' * wx0' * s0 / px (X1
  * cancels), likewise for the other texels, 0 where px == 0. */
+struct SampleEval {
+    Bilin b;
+    float sv[3][4], px[3];   /* shaded texel values, pixel colour */
+};
+
 template <bool P2>
-__device__ __forceinline__ void sample_terms(const DiffArgs &a, int32_t k, int32_t col,
-                                             size_t plane, const float run[3], uint32_t key[4],
-                                             float val[4][3])
+__device__ __forceinline__ void sample_eval(const DiffArgs &a, int32_t k, int32_t col,
+                                            size_t plane, SampleEval &e)
 {
-    Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
-    float sv[3][4], px[3];
-    sample_colour(a, b, plane, sv, px);
+    e.b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
+    sample_colour(a, e.b, plane, e.sv, e.px);
+}
+
+__device__ __forceinline__ void sample_grad(const DiffArgs &a, const SampleEval &e,
+                                            const float run[3], uint32_t key[4], float val[4][3])
+{
+    const Bilin &b = e.b;
+    const float(&sv)[3][4] = e.sv;
+    const float(&px)[3] = e.px;
     const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
     float wx0n = b.wx0, wx1n = b.wx1, wy0n = b.wy0, wy1n = b.wy1;
     if (wxs != 1.0f) {            /* exact and skipped when the sum is 1 */
@@ -1947,6 +1882,16 @@ __device__ __forceinline__ void sample_terms(const DiffArgs &a, int32_t k, int32
     }
 }
 
+template <bool P2>
+__device__ __forceinline__ void sample_terms(const DiffArgs &a, int32_t k, int32_t col,
+                                             size_t plane, const float run[3], uint32_t key[4],
+                                             float val[4][3])
+{
+    SampleEval e;
+    sample_eval<P2>(a, k, col, plane, e);
+    sample_grad(a, e, run, key, val);
+}
+
 /* Adjoint.  A workgroup covers 64 adjacent columns x 16 rows; each thread
  * holds kGradRows rows of one column.  A run of equal src down the column
  * is one visible sample: its incoming gradient is summed in registers and
@@ -1962,8 +1907,17 @@ __device__ __forceinline__ void sample_terms(const DiffArgs &a, int32_t k, int32
  * several times fewer texels than it has sample-texel terms).  Short
  * per-thread row runs keep many waves in flight for the gathers.  Each
  * table entry is then added into the global f32 gradient with one atomic
- * per channel.  The sky term is summed per workgroup into one partial. */
-template <bool P2>
+ * per channel.
+ *
+ * FWD (the fused L2 step): the incoming gradient is not read but formed
+ * here from the target, g = 2 (px - target), with the colour px of each
+ * run's sample evaluated where the run starts (the same evaluation as
+ * k_diff_color's, so img and g are bit-identical to the separate calls);
+ * img and g are written only where the caller asks for them, and each
+ * workgroup leaves one partial of the loss, sum (px - target)^2, and one of
+ * the sky term.  Otherwise each workgroup leaves one partial of the sky
+ * term. */
+template <bool P2, bool FWD>
 __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
 {
     __shared__ uint32_t keys[kGradCap];
@@ -1982,8 +1936,9 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
     const int32_t H = a.f.rows, W = a.f.cols;
     const bool ok = col < W && r0 < H;
     const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
+    const float *gin = FWD ? a.tgt : a.gimg;
     int32_t src[kGradRows];
-    float g[3][kGradRows];
+    float g[3][kGradRows];     /* FWD: the target until the pixel's g replaces it */
 #pragma unroll
     for (int j = 0; j < kGradRows; j++) {
         const int32_t r = r0 + j;
@@ -1992,17 +1947,45 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
         src[j] = in ? a.src[p] : INT32_MIN;
 #pragma unroll
         for (int ch = 0; ch < 3; ch++)
-            g[ch][j] = in ? a.gimg[ch * fplane + p] : 0.0f;
+            g[ch][j] = in ? gin[ch * fplane + p] : 0.0f;
     }
     __syncthreads();
-    float sky_acc = 0.0f;
+    float sky_acc = 0.0f, loss_acc = 0.0f;
     float run[3] = {0.0f, 0.0f, 0.0f};
+    SampleEval e;              /* FWD: the current run's sample */
 #pragma unroll
     for (int j = 0; j < kGradRows; j++) {
+        const int32_t k = src[j];
+        if constexpr (FWD) {
+            const bool starts = k != INT32_MIN && (j == 0 || src[j - 1] != k);
+            if (starts && k < 0) {
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++)
+                    e.px[ch] = sky_base(ch) * a.sky_scale;
+            }
+            const bool eval = starts && k >= 0;
+            if (__ballot(eval)) {
+                if (eval)
+                    sample_eval<P2>(a, k, col, plane, e);
+            }
+            if (k != INT32_MIN) {
+                const size_t p = (size_t)(r0 + j) * W + col;
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    const float d = e.px[ch] - g[ch][j];
+                    const float gd = 2.0f * d;
+                    if (a.img)
+                        a.img[ch * fplane + p] = e.px[ch];
+                    if (a.gout)
+                        a.gout[ch * fplane + p] = gd;
+                    loss_acc += d * d;
+                    g[ch][j] = gd;
+                }
+            }
+        }
 #pragma unroll
         for (int ch = 0; ch < 3; ch++)
             run[ch] += g[ch][j];
-        const int32_t k = src[j];
         const bool ends = (j == kGradRows - 1 || src[j + 1] != k) && k != INT32_MIN;
         if (ends && k < 0) {
 #pragma unroll
@@ -2016,8 +1999,12 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
 #pragma unroll
             for (int t = 0; t < 4; t++)
                 val[t][0] = val[t][1] = val[t][2] = 0.0f;
-            if (flush)
-                sample_terms<P2>(a, k, col, plane, run, key, val);
+            if (flush) {
+                if constexpr (FWD)
+                    sample_grad(a, e, run, key, val);
+                else
+                    sample_terms<P2>(a, k, col, plane, run, key, val);
+            }
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const bool head = segment_sum3(key[t], val[t]);
@@ -2035,26 +2022,34 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
             run[0] = run[1] = run[2] = 0.0f;
         }
     }
-    /* the fused step's loss: its partials are complete (the forward's
-     * launch has ended), one workgroup sums them */
-    if (a.loss_n > 0 && blockIdx.x == 0 && blockIdx.y == 0) {
-        block_sum_partials<kGradThreads>(a.loss_part, a.loss_n, a.loss, a.loss_accumulate);
-        __syncthreads();
-        block_sum_partials<kGradThreads>(a.sky_part, a.loss_n, a.gsky, a.loss_accumulate);
-    }
-    __shared__ float wsky[kGradThreads / 64];
+    /* per-workgroup partials, summed in a fixed order by the next launch */
+    __shared__ float wsky[kGradThreads / 64], wloss[kGradThreads / 64];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1) {
         sky_acc += __shfl_xor(sky_acc, off);
-    if ((threadIdx.x & 63) == 0)
+        if (FWD)
+            loss_acc += __shfl_xor(loss_acc, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
         wsky[threadIdx.x >> 6] = sky_acc;
+        if (FWD)
+            wloss[threadIdx.x >> 6] = loss_acc;
+    }
     __syncthreads();
-    /* the fused step took the sky term from its forward pass */
-    if (threadIdx.x == 0 && a.loss_n == 0) {
-        float t = 0.0f;
-        for (int w = 0; w < kGradThreads / 64; w++)
+    if (threadIdx.x == 0) {
+        const int32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
+        float t = 0.0f, tl = 0.0f;
+        for (int w = 0; w < kGradThreads / 64; w++) {
             t += wsky[w];
-        a.partials[blockIdx.y * gridDim.x + blockIdx.x] = t;
+            if (FWD)
+                tl += wloss[w];
+        }
+        if (FWD) {
+            a.sky_part[blk] = t;
+            a.partials[blk] = tl;
+        } else {
+            a.partials[blk] = t;
+        }
     }
     for (int32_t i = threadIdx.x; i < kGradCap; i += kGradThreads) {
         const uint32_t key = keys[i];
@@ -2067,6 +2062,18 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
                 unsafeAtomicAdd(&a.gcp[ch * plane + key], v);
         }
     }
+}
+
+/* The fused step's two sums: its loss partials into *loss and its sky
+ * partials into *gsky (= or +=), each in a fixed order. */
+__global__ __launch_bounds__(1024) void k_sum_step(const float *__restrict__ loss_part,
+                                                   const float *__restrict__ sky_part, int32_t n,
+                                                   float *__restrict__ loss, int32_t loss_acc,
+                                                   float *__restrict__ gsky, int32_t sky_acc)
+{
+    block_sum_partials<1024>(loss_part, n, loss, loss_acc);
+    __syncthreads();
+    block_sum_partials<1024>(sky_part, n, gsky, sky_acc);
 }
 
 /* L2 loss of the fitting harness: g = 2 * (img - target) and one partial
@@ -2331,10 +2338,8 @@ static DiffArgs diff_args(const fsk_frame *f, const float *cp, const float *sf, 
     a.gcp = nullptr;
     a.gsky = nullptr;
     a.partials = nullptr;
-    a.loss_part = nullptr;
-    a.loss_n = 0;
-    a.loss = nullptr;
-    a.loss_accumulate = 0;
+    a.tgt = nullptr;
+    a.gout = nullptr;
     a.sky_part = nullptr;
     return a;
 }
@@ -2423,16 +2428,6 @@ int fsk_diff_grad(const fsk_frame *f, const float *d_cp, const float *d_sf, int3
                   float *d_gsky, int32_t sky_accumulate, float *d_partials, int32_t n_partials,
                   hipStream_t s)
 {
-    return fsk_diff_grad_l2(f, d_cp, d_sf, hc, wc, d_src, d_gimg, d_gcp, d_gsky, sky_accumulate,
-                            d_partials, n_partials, nullptr, 0, nullptr, 0, nullptr, s);
-}
-
-int fsk_diff_grad_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
-                     int32_t wc, const int32_t *d_src, const float *d_gimg, float *d_gcp,
-                     float *d_gsky, int32_t sky_accumulate, float *d_partials,
-                     int32_t n_partials, const float *d_loss_part, int32_t loss_n, float *d_loss,
-                     int32_t loss_accumulate, const float *d_sky_part, hipStream_t s)
-{
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
     DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
@@ -2440,72 +2435,43 @@ int fsk_diff_grad_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, i
     a.gcp = d_gcp;
     a.gsky = d_gsky;
     a.partials = d_partials;
-    a.loss_part = d_loss_part;
-    a.loss_n = loss_n;
-    a.loss = d_loss;
-    a.loss_accumulate = loss_accumulate;
-    a.sky_part = (float *)d_sky_part;
     dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
-    /* the sums of a fused step's forward (loss_n > 0) leave no partials
-     * of the adjoint's own */
-    if (loss_n == 0 && (int64_t)grid.x * grid.y > n_partials)
+    if ((int64_t)grid.x * grid.y > n_partials)
         return hipErrorInvalidValue;
-    void (*kern)(DiffArgs) = pow2(hc) && pow2(wc) ? k_diff_grad<true> : k_diff_grad<false>;
+    void (*kern)(DiffArgs) =
+        pow2(hc) && pow2(wc) ? k_diff_grad<true, false> : k_diff_grad<false, false>;
     hipLaunchKernelGGL(kern, grid, dim3(kGradThreads), 0, s, a);
-    if (loss_n == 0)
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials,
-                           (int32_t)(grid.x * grid.y), d_gsky, sky_accumulate);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials,
+                       (int32_t)(grid.x * grid.y), d_gsky, sky_accumulate);
     return hipGetLastError();
 }
 
-int fsk_diff_color_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
-                      int32_t wc, float sky_scale, const int32_t *d_src, const float *d_tgt,
-                      float *d_img, float *d_g, float *d_loss, int32_t loss_accumulate,
-                      float *d_zero, size_t zero_n, float *d_partials, int32_t n_partials,
-                      int32_t *n_loss_partials, float *d_sky_part, hipStream_t s)
+int fsk_diff_step_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, int32_t hc,
+                     int32_t wc, float sky_scale, const int32_t *d_src, const float *d_tgt,
+                     float *d_img, float *d_g, float *d_gcp, float *d_gsky,
+                     int32_t sky_accumulate, float *d_loss, int32_t loss_accumulate,
+                     float *d_loss_part, float *d_sky_part, int32_t n_partials, hipStream_t s)
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
     DiffArgs a = diff_args(f, d_cp, d_sf, hc, wc, d_src);
-    a.sky_part = d_sky_part;
     a.sky_scale = sky_scale;
+    a.tgt = d_tgt;
     a.img = d_img;
-    a.partials = d_partials;
-    const int32_t nbx = (f->cols + 63) / 64, nby = (f->rows + kStepRows - 1) / kStepRows;
-    const int64_t npix = (int64_t)nbx * nby;
-    if (npix > n_partials)
+    a.gout = d_g;
+    a.gcp = d_gcp;
+    a.gsky = d_gsky;
+    a.partials = d_loss_part;
+    a.sky_part = d_sky_part;
+    dim3 grid((f->cols + 63) / 64, (f->rows + kGradBlockRows - 1) / kGradBlockRows);
+    const int64_t nb = (int64_t)grid.x * grid.y;
+    if (nb > n_partials)
         return hipErrorInvalidValue;
-    /* the clear rides along in float4 stores when it can; else it is a
-     * launch of its own */
-    const bool fused = d_zero && (((uintptr_t)d_zero) & 15u) == 0 && zero_n % 4 == 0;
-    size_t nzero = 0;
-    if (d_zero && !fused) {
-        int e = fsk_zero_f32(d_zero, zero_n, s);
-        if (e != hipSuccess)
-            return e;
-    } else if (fused) {
-        nzero = (zero_n / 4 + 4095) / 4096;
-        if (nzero > 2048)
-            nzero = 2048;
-        if (nzero == 0)
-            nzero = 1;
-    }
-    const int64_t total = npix + (int64_t)nzero;
-    const int32_t every = nzero ? (int32_t)(total / (int64_t)nzero) : (int32_t)total + 1;
-    const dim3 grid((unsigned)total);
-    float4 *z = fused ? (float4 *)d_zero : nullptr;
-    const size_t zn4 = fused ? zero_n / 4 : 0;
-    if (pow2(hc) && pow2(wc))
-        hipLaunchKernelGGL(k_diff_color_l2<true>, grid, dim3(1024), 0, s, a, d_tgt, d_g, nbx,
-                           (int32_t)nzero, every, z, zn4);
-    else
-        hipLaunchKernelGGL(k_diff_color_l2<false>, grid, dim3(1024), 0, s, a, d_tgt, d_g, nbx,
-                           (int32_t)nzero, every, z, zn4);
-    if (n_loss_partials)
-        *n_loss_partials = (int32_t)npix;    /* the caller sums them */
-    else
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials, (int32_t)npix,
-                           d_loss, loss_accumulate);
+    void (*kern)(DiffArgs) =
+        pow2(hc) && pow2(wc) ? k_diff_grad<true, true> : k_diff_grad<false, true>;
+    hipLaunchKernelGGL(kern, grid, dim3(kGradThreads), 0, s, a);
+    hipLaunchKernelGGL(k_sum_step, dim3(1), dim3(1024), 0, s, d_loss_part, d_sky_part,
+                       (int32_t)nb, d_loss, loss_accumulate, d_gsky, sky_accumulate);
     return hipGetLastError();
 }
 

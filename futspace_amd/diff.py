@@ -131,19 +131,23 @@ class DiffRenderer:
         return grad, loss
 
     def l2_step(self, planes, sky_scale, view: View, target, img=None, src=None, gimg=None,
-                gplanes=None, gsky=None, loss=None, accumulate=False):
+                gplanes=None, gsky=None, loss=None, accumulate=False, outputs=True):
         """One fitting step in one call: forward_raw, l2_loss_grad against
         target and vjp_raw with that gradient (fsx_render_f32_l2_step).
         Returns (img, src, gplanes, gsky, loss); with accumulate the loss and
-        the gradients are added into the given tensors."""
+        the gradients are added into the given tensors.  With outputs=False
+        the image and its loss gradient are neither allocated nor written
+        (img and gimg are ignored and None is returned for img)."""
         self._check_planes(planes)
         if tuple(target.shape) != (3, view.h, view.w) or not target.is_contiguous():
             raise ValueError("target must be a contiguous [3][h][w] tensor")
-        if img is None:
+        if not outputs:
+            img = gimg = None
+        elif img is None:
             img = torch.empty((3, view.h, view.w), dtype=torch.float32, device="cuda")
         if src is None:
             src = torch.empty((view.h, view.w), dtype=torch.int32, device="cuda")
-        if gimg is None:
+        if outputs and gimg is None:
             gimg = torch.empty_like(img)
         if gplanes is None or gsky is None or loss is None:
             accumulate = False
@@ -158,8 +162,8 @@ class DiffRenderer:
             self.ctx.check(self.ctx.lib.fsx_render_f32_l2_step(
                 self.ctx.ptr, self.map.ptr, ctypes.byref(cv), _ptr(planes), _ptr(self.sf),
                 self.hc, self.wc, ctypes.c_float(sky_scale), _ptr(target),
-                1 if accumulate else 0, _ptr(img), _ptr(src), _ptr(gimg), _ptr(gplanes),
-                _ptr(gsky), _ptr(loss)), "fsx_render_f32_l2_step")
+                1 if accumulate else 0, _ptr_or_null(img), _ptr(src), _ptr_or_null(gimg),
+                _ptr(gplanes), _ptr(gsky), _ptr(loss)), "fsx_render_f32_l2_step")
         return img, src, gplanes, gsky, loss
 
     def render(self, planes, shading, view: View):
@@ -220,19 +224,18 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
         opt = torch.optim.Adam([planes], lr=step)
     grad = torch.zeros_like(planes)
     gsky = torch.zeros(1, dtype=torch.float32, device="cuda")
-    img = gimg = src = None
+    src = None
     losses = torch.zeros(steps, dtype=torch.float32, device="cuda")
     for it in range(steps):
         # one ordering of the renderer's stream against torch's per step:
         # the views' calls queue back to back, Adam follows them
         with renderer.stream():
             for i, (v, t) in enumerate(zip(views, targets)):
-                if img is None or tuple(img.shape) != (3, v.h, v.w):
-                    img = torch.empty((3, v.h, v.w), dtype=torch.float32, device="cuda")
-                    gimg = torch.empty_like(img)
+                if src is None or tuple(src.shape) != (v.h, v.w):
                     src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
-                renderer.l2_step(planes, sky_scale, v, t, img, src, gimg, grad, gsky,
-                                 losses[it:it + 1], accumulate=i > 0)
+                # the loop needs neither the image nor its loss gradient
+                renderer.l2_step(planes, sky_scale, v, t, None, src, None, grad, gsky,
+                                 losses[it:it + 1], accumulate=i > 0, outputs=False)
         planes.grad = grad
         opt.step()
         planes.clamp_(0.0, 1.0)
@@ -243,3 +246,7 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _ptr_or_null(t):
+    return ctypes.c_void_p(None) if t is None else ctypes.c_void_p(t.data_ptr())

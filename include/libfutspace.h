@@ -250,17 +250,20 @@ int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *
 /* One fitting step of the differentiable renderer in one call: the same
  * results as fsx_render_f32, then fsx_l2_loss against dev_target[3][h][w]
  * (dev_gimg = 2 (img - target), *dev_loss), then fsx_render_f32_vjp with
- * that gradient.  accumulate = 0 overwrites *dev_loss, dev_gplanes and
- * *dev_gsky, else adds to them.  Three launches: the visibility render
- * (which also clears the gradient on CUs its tiles leave idle, else the
- * colour pass does), the colour and loss pass, and the adjoint (which
- * first sums the loss and sky partials).  The loss is summed per workgroup
- * in a fixed order, so the loss of a step is reproducible; the gradients
- * carry the f32-atomic summation order of fsx_render_f32_vjp.  The loss
- * and sky partial sums (one each per 64 x 16 pixel block) limit the frame
- * to ceil(w/64) * ceil(h/16) <= 16384 blocks (4096 x 4096, or 8192 columns
- * by 2048 rows); larger frames are refused before anything is enqueued.
- * Asynchronous on the context stream. */
+ * that gradient.  dev_img and dev_gimg may be NULL when the caller does not
+ * need them (a fitting loop needs neither); dev_src is always written.
+ * accumulate = 0 overwrites *dev_loss, dev_gplanes and *dev_gsky, else
+ * adds to them.  Three launches: the visibility render (which also clears
+ * the gradient on CUs its tiles leave idle, else a clear launch precedes
+ * the adjoint), the adjoint, which evaluates each visible sample's colour
+ * once and forms the image, the loss terms and the gradient from it, and a
+ * one-workgroup sum of the loss and sky partials.  The loss is summed per
+ * workgroup in a fixed order, so the loss of a step is reproducible; the
+ * gradients carry the f32-atomic summation order of fsx_render_f32_vjp.
+ * The loss and sky partial sums (one each per 64 x 16 pixel block) limit
+ * the frame to ceil(w/64) * ceil(h/16) <= 16384 blocks (4096 x 4096, or
+ * 8192 columns by 2048 rows); larger frames are refused before anything is
+ * enqueued.  Asynchronous on the context stream. */
 int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
                            const struct fsx_view *view, const float *dev_planes,
                            const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,

@@ -266,6 +266,30 @@ def test_fused_step_matches_separate_calls(ctx, vi):
     assert abs(float(f_gs.cpu()[0]) - want_gs) <= 1e-5 * max(sky_terms, 1e-3)
 
 
+@pytest.mark.parametrize("vi", [0, 3])
+def test_fused_step_without_image_outputs(ctx, vi):
+    """outputs=False (NULL img and dL/dimg) leaves those buffers untouched
+    and gives the same src and loss bit for bit, the same gradients within
+    the f32-atomic order, as the step that writes them."""
+    r, col, _, _ = _setup(ctx)
+    v = VIEWS[vi]
+    tp = torch.from_numpy(planes_of(col)).cuda()
+    rng = np.random.default_rng(300 + vi)
+    target = torch.from_numpy(rng.random((3, v.h, v.w), dtype=np.float32)).cuda()
+    img, src, gp, gs, loss = r.l2_step(tp, 0.37, v, target)
+    src2 = torch.full_like(src, 5)
+    gp2 = torch.full_like(tp, 9.0)
+    gs2 = torch.full((1,), 4.0, device="cuda")
+    loss2 = torch.full((1,), 2.0, device="cuda")
+    out = r.l2_step(tp, 0.37, v, target, None, src2, None, gp2, gs2, loss2, outputs=False)
+    torch.cuda.synchronize()
+    assert out[0] is None
+    assert torch.equal(src2, src)
+    assert float(loss2[0]) == float(loss[0])
+    assert _rel_err(gp2.cpu().numpy(), gp.cpu().numpy()) <= 1e-5
+    assert abs(float(gs2[0]) - float(gs[0])) <= 1e-5 * max(abs(float(gs[0])), 1.0)
+
+
 def test_fused_step_accumulates(ctx):
     """accumulate adds the loss and both gradients to the given values."""
     r, col, _, _ = _setup(ctx)
