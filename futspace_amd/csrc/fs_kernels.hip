@@ -1713,11 +1713,13 @@ __device__ __forceinline__ float sky_base(int ch)
     return ch == 2 ? 224.0f / 255.0f : 144.0f / 255.0f;
 }
 
-/* Colour of one visible sample, per channel (the relaxed png_color_filtered). */
-__device__ __forceinline__ void sample_colour(const DiffArgs &a, const Bilin &b, size_t plane,
-                                              float sv[3][4], float px[3])
+/* The shaded texel values of a sample's quad: sf[t] and the three planes'
+ * cv[ch][t].  All sixteen gathers are issued here, before any mix: the
+ * mixes branch, and loads are not moved above a branch, so gathers
+ * interleaved with them would wait for each other. */
+__device__ __forceinline__ void sample_gather(const DiffArgs &a, const Bilin &b, size_t plane,
+                                              float cv[3][4], float sf[4])
 {
-    float sf[4];
 #pragma unroll
     for (int t = 0; t < 4; t++)
         sf[t] = a.sf[b.t[t]];
@@ -1726,7 +1728,20 @@ __device__ __forceinline__ void sample_colour(const DiffArgs &a, const Bilin &b,
         const float *P = a.cp + ch * plane;
 #pragma unroll
         for (int t = 0; t < 4; t++)
-            sv[ch][t] = P[b.t[t]] * sf[t];
+            cv[ch][t] = P[b.t[t]];
+    }
+}
+
+/* The colour of a visible sample from its gathered quad, per channel (the
+ * relaxed png_color_filtered); sv[ch][t] = cv[ch][t] * sf[t]. */
+__device__ __forceinline__ void sample_mix(const Bilin &b, const float cv[3][4],
+                                           const float sf[4], float sv[3][4], float px[3])
+{
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            sv[ch][t] = cv[ch][t] * sf[t];
         float X1 = mixf(b.wx0, sv[ch][0], b.wx1, sv[ch][1]);
         float X2 = mixf(b.wx0, sv[ch][2], b.wx1, sv[ch][3]);
         px[ch] = mixf(b.wy0, X1, b.wy1, X2);
@@ -1734,7 +1749,12 @@ __device__ __forceinline__ void sample_colour(const DiffArgs &a, const Bilin &b,
 }
 
 /* Forward colour, one thread per pixel (2-D blocks of 64 columns x 4
- * rows: the gathers of neighbouring pixels share texels). */
+ * rows: the gathers of neighbouring pixels share texels).  This kernel
+ * gathers each channel's quad just before its mixes: with one pixel per
+ * thread and many waves in flight that measured faster than issuing all
+ * sixteen gathers first (which needs more registers per wave), and so did
+ * one pixel per thread against two or four, and against listing each
+ * column's runs to colour each visible sample once. */
 template <bool P2>
 __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
 {
@@ -1752,8 +1772,22 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
         for (int ch = 0; ch < 3; ch++)
             px[ch] = sky_base(ch) * a.sky_scale;
     } else {
-        float sv[3][4];
-        sample_colour(a, bilin_of<P2>(a.f, k, col, a.hc, a.wc), plane, sv, px);
+        const Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
+        float sf[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            sf[t] = a.sf[b.t[t]];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float *P = a.cp + ch * plane;
+            float sv[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                sv[t] = P[b.t[t]] * sf[t];
+            float X1 = mixf(b.wx0, sv[0], b.wx1, sv[1]);
+            float X2 = mixf(b.wx0, sv[2], b.wx1, sv[3]);
+            px[ch] = mixf(b.wy0, X1, b.wy1, X2);
+        }
     }
 #pragma unroll
     for (int ch = 0; ch < 3; ch++)
@@ -1840,6 +1874,7 @@ __device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uin
  * cancels), likewise for the other texels, 0 where px == 0. */
 struct SampleEval {
     Bilin b;
+    float sf[4];             /* shade factors of the quad */
     float sv[3][4], px[3];   /* shaded texel values, pixel colour */
 };
 
@@ -1848,7 +1883,9 @@ __device__ __forceinline__ void sample_eval(const DiffArgs &a, int32_t k, int32_
                                             size_t plane, SampleEval &e)
 {
     e.b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
-    sample_colour(a, e.b, plane, e.sv, e.px);
+    float cv[3][4];
+    sample_gather(a, e.b, plane, cv, e.sf);
+    sample_mix(e.b, cv, e.sf, e.sv, e.px);
 }
 
 __device__ __forceinline__ void sample_grad(const DiffArgs &a, const SampleEval &e,
@@ -1874,7 +1911,7 @@ __device__ __forceinline__ void sample_grad(const DiffArgs &a, const SampleEval 
         q[ch] = px[ch] > 0.0f ? run[ch] / px[ch] : 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const float sf = a.sf[b.t[t]];
+        const float sf = e.sf[t];
         key[t] = b.t[t];
 #pragma unroll
         for (int ch = 0; ch < 3; ch++)
