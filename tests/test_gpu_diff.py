@@ -366,8 +366,8 @@ def test_forward_tall_frame(ctx):
 def test_fused_step_gradient_clear(ctx, w):
     """The fused step clears stale gradient planes either in spare
     workgroups of the slice render (a narrow view leaves most CUs idle) or
-    inside the colour launch (a 1600-column view leaves too few): both
-    give the separate calls' result."""
+    in a launch of its own (a 1600-column view leaves too few): both give
+    the separate calls' result."""
     r, col, _, _ = _setup(ctx)
     v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=0.0, distance=300.0, fov=1.2,
              h=40, w=w, d=0.01, z0=0.5, scale=64.0)
