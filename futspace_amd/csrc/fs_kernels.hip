@@ -311,7 +311,17 @@ __host__ __device__ constexpr int tab_stride(int chunk)
 {
     return chunk;
 }
-constexpr int kMaxRows = 4096;                   /* queue entries keep the row in 12 bits */
+/* A queued visible sample: its slice group g, its slice j within the
+ * group, its column c in the tile and its projected row, packed as
+ * g << 22 | j << 17 | c << 13 | row (g < 256, j < 32, c < 16, row < 8192).
+ * The long-span list packs c << 26 | first row << 13 | end row. */
+constexpr uint32_t kQRowBits = 13;
+constexpr uint32_t kQRowMask = (1u << kQRowBits) - 1u;
+constexpr uint32_t kQColShift = 13, kQSliceShift = 17, kQGroupShift = 22;
+constexpr uint32_t kQLaneMask = (0xFFu << kQGroupShift) | (15u << kQColShift);  /* g and c */
+/* Frames up to kMaxRows rows: the row fields allow 8191, and a 4-column
+ * tile of that height still fits in LDS. */
+constexpr int kMaxRows = 7680;
 constexpr int kChunkAlign = 4;                   /* chunks are multiples of SG * kChunkAlign */
 
 /* blockIdx -> column tile.  Workgroups are dispatched round-robin over
@@ -874,8 +884,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
             for (int j = 0; j < KR; j++) {
                 if (vbits & (1u << j)) {
                     int32_t aj = (int32_t)((ah[j / 2] >> ((j & 1) * 16)) & 0xFFFFu);
-                    queue[slot] = ((uint32_t)g << 21) | ((uint32_t)j << 16) |
-                                  ((uint32_t)c << 12) | (uint32_t)aj;
+                    queue[slot] = ((uint32_t)g << kQGroupShift) | ((uint32_t)j << kQSliceShift) |
+                                  ((uint32_t)c << kQColShift) | (uint32_t)aj;
                     slot++;
                 }
             }
@@ -902,8 +912,9 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 #pragma unroll
             for (int u = 0; u < CBATCH; u++) {
                 uint32_t e = queue[min(i0 + (uint32_t)(u * NT + tid), nq - 1u)];
-                int32_t cc = (int32_t)((e >> 12) & 15u);
-                float4 L = tl[(int32_t)((e >> 16) & 31u) * SG + (int32_t)(e >> 21)];
+                int32_t cc = (int32_t)((e >> kQColShift) & 15u);
+                float4 L = tl[(int32_t)((e >> kQSliceShift) & 31u) * SG +
+                              (int32_t)(e >> kQGroupShift)];
                 float fc = (float)(col0 + cc);
                 es[u] = e;
                 xs[u] = L.x + (fc * L.z);
@@ -917,9 +928,9 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 if (i >= nq)
                     break;
                 const uint32_t e = es[u];
-                int32_t s = (int32_t)(e >> 21) * kr + (int32_t)((e >> 16) & 31u);
-                int32_t cc = (int32_t)((e >> 12) & 15u);
-                int32_t row = (int32_t)(e & 0xFFFu);
+                int32_t s = (int32_t)(e >> kQGroupShift) * kr + (int32_t)((e >> kQSliceShift) & 31u);
+                int32_t cc = (int32_t)((e >> kQColShift) & 15u);
+                int32_t row = (int32_t)(e & kQRowMask);
                 const float x = xs[u], y = ys[u];
                 uint32_t colour;
                 if (MODE == kOutColour)
@@ -936,16 +947,16 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
                 /* the entry before, if it is this lane's (same g and c),
                  * holds the row where this sample's span ends */
                 const uint32_t ep = queue[i > 0 ? i - 1u : 0u];
-                const bool first = i == 0 || ((e ^ ep) & 0x1FE0F000u) != 0;
-                int32_t bottom = first ? (int32_t)lanein[(int32_t)(e >> 21) * TC + cc]
-                                       : (int32_t)(ep & 0xFFFu);
+                const bool first = i == 0 || ((e ^ ep) & kQLaneMask) != 0;
+                int32_t bottom = first ? (int32_t)lanein[(int32_t)(e >> kQGroupShift) * TC + cc]
+                                       : (int32_t)(ep & kQRowMask);
                 uint32_t *dst = stripe + cc * Hp;
                 int32_t stop = min(bottom, row + kShortSpan);
                 for (int32_t r = row; r < stop; r++)
                     dst[r] = colour;
                 if (bottom > stop) {
                     uint32_t li = atomicAdd(&sh.nlong, 1u);
-                    longs[2 * li] = ((uint32_t)cc << 26) | ((uint32_t)stop << 13) |
+                    longs[2 * li] = ((uint32_t)cc << 26) | ((uint32_t)stop << kQRowBits) |
                                     (uint32_t)bottom;
                     longs[2 * li + 1] = colour;
                 }
@@ -964,8 +975,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
     const uint32_t nl = sh.nlong;
     for (uint32_t li = (uint32_t)wave; li < nl; li += NW) {
         uint32_t d = longs[2 * li], colour = longs[2 * li + 1];
-        int32_t cc = (int32_t)(d >> 26), r0 = (int32_t)((d >> 13) & 0x1FFFu);
-        int32_t r1 = (int32_t)(d & 0x1FFFu);
+        int32_t cc = (int32_t)(d >> 26), r0 = (int32_t)((d >> kQRowBits) & kQRowMask);
+        int32_t r1 = (int32_t)(d & kQRowMask);
         for (int32_t r = r0 + lane; r < r1; r += 64)
             stripe[cc * Hp + r] = colour;
     }
@@ -1455,13 +1466,23 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
     }
     /* a chunk holds whole groups of kChunkAlign slices per slice group;
      * tall frames take shorter chunks until the tile fits in LDS */
-    int ga = (s.nt / s.tc) * kChunkAlign;
-    s.chunk = s.chunk < ga ? ga : s.chunk - s.chunk % ga;
     auto need = [&](int c) {
         return render_lds_bytes(s.tc, s.nt, rows, c) + kStaticLds;
     };
-    while (s.chunk > ga && (size_t)s.wg * need(s.chunk) > lds_per_cu())
-        s.chunk = (s.chunk / 2) - (s.chunk / 2) % ga;
+    auto fit = [&]() {
+        const int ga = (s.nt / s.tc) * kChunkAlign;
+        s.chunk = s.chunk < ga ? ga : s.chunk - s.chunk % ga;
+        while (s.chunk > ga && (size_t)s.wg * need(s.chunk) > lds_per_cu())
+            s.chunk = (s.chunk / 2) - (s.chunk / 2) % ga;
+        return (size_t)s.wg * need(s.chunk) <= lds_per_cu();
+    };
+    if (!fit() && s.tc == 4 && s.nt == 1024) {
+        /* the tallest frames: 512-thread tiles, whose chunks go down to
+         * 512 slices */
+        s.nt = 512;
+        s.wg = 1;
+        fit();
+    }
     return s;
 }
 

@@ -126,7 +126,7 @@ int futhark_entry_update_map(struct futhark_context *ctx, struct futhark_opaque_
  * hard-codes -- d (invz_param1, fut/voxel_renderer.fut:104), z0
  * (invz_param2, :103), scale (magic_number, :217; <= 0 means f32(w/2)),
  * n_slices (<= 0 means get_zs' count from distance, :28-34; a render
- * refuses more than 2^24 slices, and frames of more than 4096 rows).
+ * refuses more than 2^24 slices, and frames of more than 7680 rows).
  * `map` indexes the map list of a batched render. */
 struct fsx_view {
     float x, y, height, angle, horizon, distance, fov;
@@ -227,7 +227,8 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
  * the forward pass; accumulate = 0 clears both first.  f32 atomics: the
  * summation order, hence the last bits, varies from run to run.  Its
  * per-workgroup partial sums limit the frame to ceil(w/64) * ceil(h/16)
- * <= 32768 blocks of 64 x 16 pixels (h <= 4096 rows: every w up to 8192);
+ * <= 32768 blocks of 64 x 16 pixels (h <= 4096 rows: every w up to 8192;
+ * h <= 7680: w up to 4352);
  * larger frames are refused before anything is enqueued.
  *
  * All three are asynchronous on the context stream. */

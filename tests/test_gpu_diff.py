@@ -82,6 +82,20 @@ def test_forward_sixteen_bit_heights(ctx):
     assert np.array_equal(img.cpu().numpy(), want_img)
 
 
+def test_forward_tall_frame(ctx):
+    """An 8K-tall (4320-row) frame: 4-column slice-mode tiles, bit-exact."""
+    r, col, alt, sf = _setup(ctx)
+    v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=1700.0, distance=300.0, fov=1.2,
+             h=4320, w=20, d=0.01, z0=0.5, scale=1400.0)
+    planes = planes_of(col)
+    img, src = r.forward_raw(torch.from_numpy(planes).cuda(), 0.37, v)
+    want_img, want_src = O.render_f32(v, planes, sf, alt, 0.37)
+    got_src = src.cpu().numpy()
+    cols = np.arange(v.w, dtype=np.int32)[None, :]
+    assert np.array_equal(np.where(got_src < 0, -1, got_src * v.w + cols), want_src)
+    assert np.array_equal(img.cpu().numpy().view(np.uint32), want_img.view(np.uint32))
+
+
 def _rel_err(got, want):
     scale = max(float(np.abs(want).max()), 1e-30)
     return float(np.abs(got.astype(np.float64) - want.astype(np.float64)).max()) / scale

@@ -209,10 +209,11 @@ def test_kernel_shapes_bitexact(ctx, small_scene, shape):
         ctx.set_tuning(0, 0, 0)
 
 
-@pytest.mark.parametrize("h,w", [(2160, 72), (4096, 24)])
+@pytest.mark.parametrize("h,w", [(2160, 72), (4096, 24), (4320, 20), (7680, 9)])
 def test_tall_frames(ctx, small_scene, h, w):
     """Frames whose 8-column stripe does not fit in LDS take 4-column
-    tiles (and, at 4096 rows, the shortest chunks): still bit-exact."""
+    tiles (with shorter chunks as the frame grows; the tallest, up to the
+    7680-row limit, 512-thread tiles): still bit-exact."""
     col, hts = small_scene
     m = ctx.map(col, hts)
     shaded, alt = O.update_map(col, hts)
@@ -225,7 +226,7 @@ def test_rows_beyond_the_limit_are_rejected(ctx, small_scene):
     col, hts = small_scene
     m = ctx.map(col, hts)
     v = View(x=40.3, y=90.6, height=230.0, angle=1.3, horizon=100.0, distance=400.0, fov=1.2,
-             h=4097, w=8, d=0.004, z0=0.5)
+             h=7681, w=8, d=0.004, z0=0.5)
     with pytest.raises(RuntimeError):
         ctx.render(m, v)
 
