@@ -12,7 +12,9 @@ host loads maps (c/freeimage_futspace.h:47-53: alpha 0xFF).
 
 `python -m futspace_amd.futdata render < colour.in height.in > frame.data`
 runs the stateless program of tests/futspace.fut on the GPU, like
-`./futspace -b` in tests/Makefile:3.
+`./futspace -b` in tests/Makefile:3; `python -m futspace_amd.futdata bench`
+times it over the input sweeps of its `futhark bench` header
+(tests/futspace.fut:1-22).
 """
 import struct
 import sys
@@ -80,10 +82,76 @@ def i32_to_rgb(frame):
     return np.stack([(a >> 16) & 255, (a >> 8) & 255, a & 255], axis=-1).astype(np.uint8)
 
 
+# The sweeps of tests/futspace.fut:1-22: the program as a function of the
+# depth (camera distance, d_*), the frame's columns (m_*) and its rows
+# (l_*), each from 10 to 10^6.  The datasets themselves are not in the
+# reference tree; seeded 1024^2 maps stand in for them.
+SWEEP = (10, 100, 1000, 10000, 100000, 500000, 1000000)
+
+
+def bench_stateless(reps=10, seed=3, size=1024, out=None):
+    """Time fsx_render_stateless on `main`'s camera (tests/futspace.fut:
+    127-145) with one of distance, columns or rows swept over SWEEP (the
+    others at main's 800, 800 and 400); frames stay in device memory.
+    Prints one JSON line per case; frames taller than the library's limit
+    are reported as refused."""
+    import ctypes
+    import dataclasses
+    import json
+
+    import torch
+
+    from . import scene
+    from .renderer import STATELESS_MAIN, Context
+
+    out = out or sys.stdout
+    col, hts = scene.scene_maps(seed, size)
+    colour = torch.from_numpy((np.asarray(col, np.uint32) & 0xFFFFFF).astype(np.int32)).cuda()
+    height = torch.from_numpy((np.asarray(hts, np.uint32) & 0xFF).astype(np.int32)).cuda()
+    ctx = Context()
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream)
+    cases = ([("d", n, dataclasses.replace(STATELESS_MAIN, distance=float(n))) for n in SWEEP] +
+             [("m", n, dataclasses.replace(STATELESS_MAIN, w=n)) for n in SWEEP] +
+             [("l", n, dataclasses.replace(STATELESS_MAIN, h=n)) for n in SWEEP])
+    for name, n, v in cases:
+        frame = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
+        cv = v.to_c()
+
+        def call():
+            return ctx.lib.fsx_render_stateless(
+                ctx.ptr, ctypes.c_void_p(colour.data_ptr()), ctypes.c_void_p(height.data_ptr()),
+                size, size, ctypes.byref(cv), ctypes.c_void_p(frame.data_ptr()))
+
+        rec = {"input": f"{name}_{n}", "distance": v.distance, "rows": v.h, "cols": v.w}
+        if call() != 0:
+            rec["refused"] = ctx.error()
+        else:
+            ctx.sync()
+            with torch.cuda.stream(stream):
+                call()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(reps):
+                    call()
+                e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            rec.update({"ms_per_frame": ms, "mpixel_per_s": v.h * v.w / ms / 1e3})
+        del frame
+        out.write(json.dumps(rec) + "\n")
+        out.flush()
+    ctx.close()
+
+
 def _main(argv):
+    if len(argv) >= 2 and argv[1] == "bench":
+        bench_stateless()
+        return 0
     if len(argv) < 2 or argv[1] != "render":
         sys.stderr.write("usage: python -m futspace_amd.futdata render < colour.in height.in "
-                         "> frame.data\n")
+                         "> frame.data\n       python -m futspace_amd.futdata bench\n")
         return 2
     colour = read_value(sys.stdin.buffer)
     height = read_value(sys.stdin.buffer)

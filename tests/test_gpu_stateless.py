@@ -78,3 +78,20 @@ def test_command_line_like_futspace_b(tmp_path):
     frame = futdata.read_value(io.BytesIO(r.stdout))
     assert frame.dtype == np.int32 and frame.shape == (400, 800)
     assert np.array_equal(frame, _oracle(STATELESS_MAIN, colour, height))
+
+
+def test_bench_sweep_runs(ctx):
+    """`python -m futspace_amd.futdata bench`: every case of the sweep
+    either times a frame or, beyond the row limit, is refused with a
+    message."""
+    import io
+    import json
+    buf = io.StringIO()
+    futdata.bench_stateless(reps=1, out=buf)
+    recs = [json.loads(line) for line in buf.getvalue().splitlines()]
+    assert len(recs) == 3 * len(futdata.SWEEP)
+    for r in recs:
+        if r["rows"] <= 7680:
+            assert r["ms_per_frame"] > 0, r
+        else:
+            assert "refused" in r and "unsupported" in r["refused"], r
