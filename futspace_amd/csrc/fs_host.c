@@ -1936,7 +1936,7 @@ int fsx_batch_new(struct futhark_context *ctx, const struct fsx_map *const *maps
             set_error(ctx, "fsx_batch_new: view %d names map %d of %d", i, v->map, n_maps);
             return 1;
         }
-        if (v->h <= 0 || v->w <= 0 || v->h > fsk_render_max_rows() || v->w > 65535 * 4) {
+        if (v->h <= 0 || v->w <= 0 || v->h > fsk_batch_max_rows() || v->w > 65535 * 4) {
             set_error(ctx, "fsx_batch_new: view %d has unsupported size %d x %d", i, v->h, v->w);
             return 1;
         }

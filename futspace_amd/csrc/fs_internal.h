@@ -77,7 +77,10 @@ void fsk_default_tuning(fsk_tuning *t);
  * that t selects; the kernel builds its own slice table per depth chunk. */
 int fsk_render(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t, uint32_t *d_out,
                int64_t pitch, hipStream_t s);
+/* Frame rows a render takes (frames taller than a tile's limit take a
+ * slower one-thread-per-column kernel), and the rows of a batched view. */
 int32_t fsk_render_max_rows(void);
+int32_t fsk_batch_max_rows(void);
 
 /* One view of a batched render, as the persistent kernel reads it. */
 typedef struct {

@@ -1122,6 +1122,80 @@ __global__ __launch_bounds__(NT, WPE) void k_render(RenderArgs a)
     }
 }
 
+/* Frames taller than the tile's kMaxRows: one thread per column walks the
+ * slices front to back and paints each visible sample's span straight into
+ * the frame (the reference's scan occlude, scatter and fill_vline,
+ * fut/voxel_renderer.fut:231-249, seen from the sample's side, as the tile
+ * does), then writes the sky above the final y-buffer and, if a
+ * transparent sample occurred, fills the column top-down.  Slow -- one
+ * thread per column, spans stored one cell at a time -- but without a row
+ * limit.  Cells of transparent samples hold the mode's `empty` value until
+ * the fill: 0 in colour mode (a frame colour is never 0), the sky in the
+ * stateless program's mode (its fill's own test), 0xFFFFFFFE in slice
+ * mode (a slice index or -1 otherwise). */
+constexpr int kMaxFrameRows = 1 << 22;
+
+template <int HBITS, bool HP2, bool CP2, int MODE>
+__global__ __launch_bounds__(64) void k_render_column(RenderArgs a)
+{
+    __shared__ float sq[256];
+    if (MODE == kOutColour) {
+        for (int32_t i = threadIdx.x; i < 256; i += 64)
+            fill_square_lut(sq, i);
+        __syncthreads();
+    }
+    const fsk_frame &f = a.f;
+    const fsk_map &m = a.m;
+    const int32_t col = blockIdx.x * 64 + threadIdx.x;
+    if (col >= f.cols)
+        return;
+    const int32_t H = f.rows;
+    const float Hf = (float)H, fcol = (float)col;
+    const uint32_t sky = MODE == kOutSlice ? 0xFFFFFFFFu : f.sky;
+    const uint32_t empty = MODE == kOutColour ? 0u : (MODE == kOutNearest ? f.sky : 0xFFFFFFFEu);
+    uint32_t *dst = a.out + col;
+    const size_t pitch = (size_t)a.pitch;
+    int32_t ybuf = H;
+    bool transp = false;
+    for (int32_t k = 0; k < f.n && ybuf > 0; k++) {
+        const Slice sl = slice_params(f, k);
+        const float x = sl.sx + (fcol * sl.dx);
+        const float y = sl.sy + (fcol * sl.dy);
+        float hgt;
+        if (MODE == kOutNearest)
+            hgt = (float)((const int32_t *)m.hq)[nearest_index<HP2>(m, x, y)];
+        else
+            hgt = height_interp<HBITS>(height_quad<HBITS, HP2>(m, x, y), x, y);
+        const float rel = ((f.cam_h - hgt) * sl.iz) + f.horizon;
+        const int32_t r = screen_row_clamped(rel, Hf);
+        if (r >= ybuf)
+            continue;
+        uint32_t v;
+        if (MODE == kOutColour)
+            v = color_of_quad(color_quad<CP2>(m, x, y), sq, x, y);
+        else if (MODE == kOutNearest)
+            v = ((const uint32_t *)m.cq)[nearest_index<HP2>(m, x, y)];
+        else
+            v = (floorf(x) == ceilf(x) || floorf(y) == ceilf(y)) ? empty : (uint32_t)k;
+        transp = transp || v == empty;
+        for (int32_t rr = r; rr < ybuf; rr++)
+            dst[(size_t)rr * pitch] = v;
+        ybuf = r;
+    }
+    for (int32_t rr = 0; rr < ybuf; rr++)
+        dst[(size_t)rr * pitch] = sky;
+    if (transp) {
+        uint32_t last = sky;
+        for (int32_t rr = 0; rr < H; rr++) {
+            const uint32_t v = dst[(size_t)rr * pitch];
+            if (v == empty)
+                dst[(size_t)rr * pitch] = last;
+            else
+                last = v;
+        }
+    }
+}
+
 /* render2 (fut/voxel_renderer.fut:257-305), the reference's ray-march
  * renderer, one thread per pixel.  The reference evaluates every
  * dist = 0 ... n and keeps the hit with the smallest dist (redstuff,
@@ -1490,6 +1564,16 @@ template <int HBITS, bool HP2, bool CP2, int MODE = kOutColour>
 hipError_t launch_render(RenderArgs a, const fsk_tuning &tune, hipStream_t s,
                          int32_t *nzero = nullptr)
 {
+    if (a.f.rows > kMaxRows) {
+        /* taller than a tile: the column kernel (no gradient clear) */
+        if (nzero)
+            *nzero = 0;
+        if (a.f.rows > kMaxFrameRows)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_render_column<HBITS, HP2, CP2, MODE>), dim3((a.f.cols + 63) / 64),
+                           dim3(64), 0, s, a);
+        return hipGetLastError();
+    }
     Shape sh = resolve_shape(a.f.rows, tune);
     a.chunk_max = sh.chunk;
     a.ntiles = (a.f.cols + sh.tc - 1) / sh.tc;
@@ -2339,8 +2423,6 @@ int fsk_render_slices_clear(const fsk_map *m, const fsk_frame *f, const fsk_tuni
         *cleared = 0;
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
-    if (f->rows > kMaxRows)
-        return hipErrorInvalidValue;
     RenderArgs a;
     a.m = *m;
     a.f = *f;
@@ -2407,8 +2489,6 @@ int fsk_render_nearest(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
-    if (f->rows > kMaxRows)
-        return hipErrorInvalidValue;
     RenderArgs a;
     a.m = *m;
     a.f = *f;
@@ -2602,6 +2682,11 @@ int32_t fsk_frame_tame(const fsk_frame *f)
 
 int32_t fsk_render_max_rows(void)
 {
+    return kMaxFrameRows;
+}
+
+int32_t fsk_batch_max_rows(void)
+{
     return kMaxRows;
 }
 
@@ -2618,8 +2703,6 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t, uint32
 {
     if (f->rows <= 0 || f->cols <= 0)
         return hipSuccess;
-    if (f->rows > kMaxRows)
-        return hipErrorInvalidValue;
     RenderArgs a;
     a.m = *m;
     a.f = *f;

@@ -126,7 +126,9 @@ int futhark_entry_update_map(struct futhark_context *ctx, struct futhark_opaque_
  * hard-codes -- d (invz_param1, fut/voxel_renderer.fut:104), z0
  * (invz_param2, :103), scale (magic_number, :217; <= 0 means f32(w/2)),
  * n_slices (<= 0 means get_zs' count from distance, :28-34; a render
- * refuses more than 2^24 slices, and frames of more than 7680 rows).
+ * refuses more than 2^24 slices, and frames of more than 2^22 rows;
+ * frames of more than 7680 rows take a slower one-thread-per-column
+ * kernel, and a batched view has at most 7680 rows).
  * `map` indexes the map list of a batched render. */
 struct fsx_view {
     float x, y, height, angle, horizon, distance, fov;
@@ -228,7 +230,7 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
  * summation order, hence the last bits, varies from run to run.  Its
  * per-workgroup partial sums limit the frame to ceil(w/64) * ceil(h/16)
  * <= 32768 blocks of 64 x 16 pixels (h <= 4096 rows: every w up to 8192;
- * h <= 7680: w up to 4352);
+ * h <= 7680: w up to 4352; taller frames, fewer columns);
  * larger frames are refused before anything is enqueued.
  *
  * All three are asynchronous on the context stream. */

@@ -165,3 +165,15 @@ def test_batch_sixteen_bit_heights(ctx):
         v.height += 600.0
     flat, offs = _run_batch(ctx, maps, views)
     _check_views(ctx, maps, views, flat, offs)
+
+
+def test_batch_refuses_views_taller_than_a_tile(ctx, pow2_maps):
+    """Batched views are tile renders: a view of more than 7680 rows is
+    refused before anything runs (single renders of it take the column
+    kernel instead)."""
+    maps, _ = pow2_maps
+    v = View(x=100.25, y=60.5, height=300.0, angle=2.4, horizon=900.0, distance=500.0, fov=1.2,
+             sky_color=0xFF9090E0, h=7681, w=8, d=0.003, z0=0.5, map=0)
+    out = torch.empty(v.h * v.w, dtype=torch.int32, device="cuda")
+    with pytest.raises(RuntimeError):
+        ctx.batch(maps, [v], out, np.array([0], dtype=np.int64))

@@ -82,11 +82,15 @@ def test_forward_sixteen_bit_heights(ctx):
     assert np.array_equal(img.cpu().numpy(), want_img)
 
 
-def test_forward_tall_frame(ctx):
-    """An 8K-tall (4320-row) frame: 4-column slice-mode tiles, bit-exact."""
+@pytest.mark.parametrize("h,w", [(4320, 20), (9000, 7)])
+def test_forward_tall_frame(ctx, h, w):
+    """An 8K-tall (4320-row) frame takes 4-column slice-mode tiles, a
+    9000-row one the column kernel: both bit-exact, and the fused step of
+    the 9000-row one (its gradient cleared in a launch of its own) gives
+    the separate calls' result."""
     r, col, alt, sf = _setup(ctx)
-    v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=1700.0, distance=300.0, fov=1.2,
-             h=4320, w=20, d=0.01, z0=0.5, scale=1400.0)
+    v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=h * 0.4, distance=300.0, fov=1.2,
+             h=h, w=w, d=0.01, z0=0.5, scale=h / 3.0)
     planes = planes_of(col)
     img, src = r.forward_raw(torch.from_numpy(planes).cuda(), 0.37, v)
     want_img, want_src = O.render_f32(v, planes, sf, alt, 0.37)
@@ -94,6 +98,17 @@ def test_forward_tall_frame(ctx):
     cols = np.arange(v.w, dtype=np.int32)[None, :]
     assert np.array_equal(np.where(got_src < 0, -1, got_src * v.w + cols), want_src)
     assert np.array_equal(img.cpu().numpy().view(np.uint32), want_img.view(np.uint32))
+    if h > 7680:
+        tp = torch.from_numpy(planes).cuda()
+        target = torch.from_numpy(np.random.default_rng(h).random((3, h, w), dtype=np.float32)).cuda()
+        gimg, loss = r.l2_loss_grad(img, target)
+        gp, gs = r.vjp_raw(tp, src, gimg, v)
+        gp2 = torch.full_like(tp, 9.0)
+        _, src2, gp2, gs2, loss2 = r.l2_step(tp, 0.37, v, target, gplanes=gp2)
+        torch.cuda.synchronize()
+        assert torch.equal(src2, src)
+        assert abs(float(loss2[0]) - float(loss[0])) <= 1e-5 * float(loss[0])
+        assert _rel_err(gp2.cpu().numpy(), gp.cpu().numpy()) <= 1e-5
 
 
 def _rel_err(got, want):

@@ -222,11 +222,26 @@ def test_tall_frames(ctx, small_scene, h, w):
     _check(ctx.render(m, v), O.render_off(v, shaded, alt))
 
 
+@pytest.mark.parametrize("h,w,seed", [(7681, 13, 0), (12000, 9, 1)])
+def test_frames_taller_than_a_tile(ctx, small_scene, h, w, seed):
+    """Frames of more than 7680 rows take the one-thread-per-column kernel:
+    bit-exact, with colour-0 (transparent) texels in the map as well."""
+    col, hts = small_scene
+    col = np.array(col, copy=True)
+    if seed:
+        col[np.random.default_rng(seed).random(col.shape) < 0.02] = 0
+    m = ctx.map(col, hts)
+    shaded, alt = O.update_map(col, hts)
+    v = View(x=40.3, y=90.6, height=230.0, angle=1.3, horizon=h * 0.4, distance=400.0, fov=1.2,
+             sky_color=sky_color(0.1), h=h, w=w, d=0.004, z0=0.5, scale=h / 3.0)
+    _check(ctx.render(m, v), O.render_off(v, shaded, alt))
+
+
 def test_rows_beyond_the_limit_are_rejected(ctx, small_scene):
     col, hts = small_scene
     m = ctx.map(col, hts)
     v = View(x=40.3, y=90.6, height=230.0, angle=1.3, horizon=100.0, distance=400.0, fov=1.2,
-             h=7681, w=8, d=0.004, z0=0.5)
+             h=(1 << 22) + 1, w=1, d=0.004, z0=0.5)
     with pytest.raises(RuntimeError):
         ctx.render(m, v)
 
