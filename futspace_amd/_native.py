@@ -58,6 +58,7 @@ I32 = ctypes.c_int32
 I64 = ctypes.c_int64
 U32 = ctypes.c_uint32
 F32 = ctypes.c_float
+F64 = ctypes.c_double
 
 # name -> (restype, argtypes); the opaque structs are passed as void*.
 FUTSPACE_SIGNATURES = {
@@ -116,6 +117,7 @@ FUTSPACE_SIGNATURES = {
     "fsx_render_f32_vjp": (ctypes.c_int, [P, ctypes.POINTER(FsxView), P, P, I32, I32, P, P, I32,
                                           P, P]),
     "fsx_l2_loss": (ctypes.c_int, [P, P, P, I64, I32, P, P]),
+    "fsx_adam_step": (ctypes.c_int, [P, P, P, P, P, I64, F64, F64, F64, F64, I32, F32, F32]),
     "fsx_render_f32_l2_step": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, P, I32, I32, F32,
                                               P, I32, P, P, P, P, P, P]),
     "fsx_render_f32_l2_step_noalloc": (ctypes.c_int, [P, P, ctypes.POINTER(FsxView), P, P, I32, I32, F32,

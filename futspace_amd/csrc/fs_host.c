@@ -1800,6 +1800,26 @@ int fsx_render_f32_l2_step_noalloc(struct futhark_context *ctx, const struct fsx
                    dev_gplanes, dev_gsky, dev_loss);
 }
 
+int fsx_adam_step(struct futhark_context *ctx, float *dev_param, const float *dev_grad,
+                  float *dev_m, float *dev_v, int64_t n, double lr, double beta1,
+                  double beta2, double eps, int32_t step, float lo, float hi)
+{
+    if (!ctx)
+        return 1;
+    if (n < 0 || step < 1 || !(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0) ||
+        ((!dev_param || !dev_grad || !dev_m || !dev_v) && n > 0) ||
+        (((uintptr_t)dev_param | (uintptr_t)dev_grad | (uintptr_t)dev_m | (uintptr_t)dev_v) &
+         15u)) {
+        set_error(ctx, "fsx_adam_step: invalid arguments (null or misaligned buffer, step < 1 "
+                       "or a beta outside [0, 1))");
+        return 1;
+    }
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    HIPCHECK(ctx, fsk_adam(dev_param, dev_grad, dev_m, dev_v, (size_t)n, lr, beta1, beta2, eps,
+                           step, lo, hi, ctx->stream));
+    return 0;
+}
+
 int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *dev_target,
                 int64_t n, int32_t accumulate, float *dev_grad, float *dev_loss)
 {

@@ -159,6 +159,11 @@ int fsk_diff_step_l2(const fsk_frame *f, const float *d_cp, const float *d_sf, i
                      int32_t sky_accumulate, float *d_loss, int32_t loss_accumulate,
                      float *d_loss_part, float *d_sky_part, int32_t n_partials, hipStream_t s);
 int fsk_zero_f32(float *d_p, size_t n, hipStream_t s);
+/* One bias-corrected Adam step (step t >= 1) on d_p[n] with gradient d_g
+ * and moments d_m, d_v, then d_p clamped to [lo, hi]. */
+int fsk_adam(float *d_p, const float *d_g, float *d_m, float *d_v, size_t n, double lr,
+             double beta1, double beta2, double eps, int32_t step, float lo, float hi,
+             hipStream_t s);
 int fsk_fill_u32(uint32_t *d_p, size_t n, uint32_t v, hipStream_t s);
 /* Reductions write one partial sum per workgroup into d_partials (at most
  * n_partials) and a one-workgroup pass adds their sum, in a fixed order,

@@ -2249,6 +2249,57 @@ __global__ __launch_bounds__(256) void k_l2_loss(const float *__restrict__ img,
         part[blockIdx.x] = (wpart[0] + wpart[1]) + (wpart[2] + wpart[3]);
 }
 
+/* One Adam step (no weight decay, bias-corrected moments) on n f32
+ * parameters, then the parameters clamped to [lo, hi]: the fitting loop's
+ * optimizer step and its projection back into the relaxation's domain in
+ * one pass over parameter, gradient and moments (float4 where aligned).
+ *   m = b1 m + (1 - b1) g,  v = b2 v + (1 - b2) g^2,
+ *   p = clamp(p - (lr / (1 - b1^t)) m / (sqrt(v) / sqrt(1 - b2^t) + eps)) */
+struct AdamArgs {
+    float lr_t;          /* lr / (1 - b1^t) */
+    float rbc2;          /* 1 / sqrt(1 - b2^t) */
+    float b1, b2, omb1, omb2, eps, lo, hi;   /* omb = 1 - b, rounded once from double */
+};
+
+__device__ __forceinline__ float adam_one(float &p, float g, float &m, float &v,
+                                          const AdamArgs &c)
+{
+    m = (c.b1 * m) + (c.omb1 * g);
+    v = (c.b2 * v) + (c.omb2 * (g * g));
+    /* the hardware sqrt and reciprocal (within an ulp or two): an
+     * optimizer step has no bit-exact counterpart, and the correctly
+     * rounded sequences would make this memory-bound pass VALU-bound */
+    const float denom = (__builtin_amdgcn_sqrtf(v) * c.rbc2) + c.eps;
+    p = fminf(fmaxf(p - (c.lr_t * (m * __builtin_amdgcn_rcpf(denom))), c.lo), c.hi);
+    return p;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v,
+                                              size_t n, AdamArgs c)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t n4 = n / 4;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        float4 pp = ((float4 *)p)[i], mm = ((float4 *)m)[i], vv = ((float4 *)v)[i];
+        const float4 gg = ((const float4 *)g)[i];
+        adam_one(pp.x, gg.x, mm.x, vv.x, c);
+        adam_one(pp.y, gg.y, mm.y, vv.y, c);
+        adam_one(pp.z, gg.z, mm.z, vv.z, c);
+        adam_one(pp.w, gg.w, mm.w, vv.w, c);
+        ((float4 *)p)[i] = pp;
+        ((float4 *)m)[i] = mm;
+        ((float4 *)v)[i] = vv;
+    }
+    for (size_t i = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float pp = p[i], mm = m[i], vv = v[i];
+        adam_one(pp, g[i], mm, vv, c);
+        p[i] = pp;
+        m[i] = mm;
+        v[i] = vv;
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_sum_partials(const float *__restrict__ part, int32_t n,
                                                        float *__restrict__ dst, int32_t accumulate)
 {
@@ -2536,6 +2587,33 @@ int fsk_l2_loss(const float *d_img, const float *d_tgt, size_t n, float *d_g, fl
                        d_partials);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, d_partials, (int32_t)blocks,
                        d_loss, accumulate);
+    return hipGetLastError();
+}
+
+int fsk_adam(float *d_p, const float *d_g, float *d_m, float *d_v, size_t n, double lr,
+             double beta1, double beta2, double eps, int32_t step, float lo, float hi,
+             hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    /* the hyper-parameters' derived constants in double, each rounded to
+     * f32 once */
+    AdamArgs c;
+    c.lr_t = (float)(lr / (1.0 - pow(beta1, (double)step)));
+    c.rbc2 = (float)(1.0 / sqrt(1.0 - pow(beta2, (double)step)));
+    c.b1 = (float)beta1;
+    c.b2 = (float)beta2;
+    c.omb1 = (float)(1.0 - beta1);
+    c.omb2 = (float)(1.0 - beta2);
+    c.eps = (float)eps;
+    c.lo = lo;
+    c.hi = hi;
+    size_t blocks = (n / 4 + 255) / 256;
+    if (blocks > 16384)
+        blocks = 16384;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, s, d_p, d_g, d_m, d_v, n, c);
     return hipGetLastError();
 }
 

@@ -166,6 +166,23 @@ class DiffRenderer:
                 _ptr(gplanes), _ptr(gsky), _ptr(loss)), "fsx_render_f32_l2_step")
         return img, src, gplanes, gsky, loss
 
+    def adam_step(self, param, grad, m, v, lr, step, betas=(0.9, 0.999), eps=1e-8, lo=0.0,
+                  hi=1.0):
+        """One Adam step (fsx_adam_step: bias-corrected, no weight decay,
+        step >= 1) on param in place, its moments m and v updated in place,
+        then param clamped to [lo, hi]; on the context stream."""
+        for t in (param, grad, m, v):
+            if (t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous()
+                    or t.shape != param.shape):
+                raise ValueError("param, grad, m and v must be contiguous CUDA float32 "
+                                 "tensors of one shape")
+        with self.stream():
+            self.ctx.check(self.ctx.lib.fsx_adam_step(
+                self.ctx.ptr, _ptr(param), _ptr(grad), _ptr(m), _ptr(v), param.numel(),
+                float(lr), float(betas[0]), float(betas[1]), float(eps), int(step),
+                ctypes.c_float(lo), ctypes.c_float(hi)),
+                "fsx_adam_step")
+
     def render(self, planes, shading, view: View):
         """Differentiable render: img[3][h][w] as a function of planes and
         shading = (sun_height, sun_ang)."""
@@ -212,33 +229,28 @@ def fit(renderer, planes, views, targets, sky_scale, steps, lr=0.5, on_step=None
 
     lr is in the colour map's own unit, one u8 channel step (the planes hold
     channel / 255), so Adam moves each plane value by about lr / 255 per
-    step; after each step the planes are clamped back to [0, 1], the domain
-    of the relaxation.  Returns the per-step losses (a CUDA tensor, read
-    once at the end)."""
+    step; each step is one pass of the library's Adam kernel, which also
+    clamps the planes back to [0, 1], the domain of the relaxation.
+    Returns the per-step losses (a CUDA tensor, read once at the end)."""
     planes.requires_grad_(False)
-    step = lr / COLOUR_UNIT
-    try:
-        # one pass over parameter, gradient and moments per step
-        opt = torch.optim.Adam([planes], lr=step, fused=True)
-    except (RuntimeError, TypeError):
-        opt = torch.optim.Adam([planes], lr=step)
+    step_lr = lr / COLOUR_UNIT
     grad = torch.zeros_like(planes)
+    m = torch.zeros_like(planes)
+    v = torch.zeros_like(planes)
     gsky = torch.zeros(1, dtype=torch.float32, device="cuda")
     src = None
     losses = torch.zeros(steps, dtype=torch.float32, device="cuda")
     for it in range(steps):
         # one ordering of the renderer's stream against torch's per step:
-        # the views' calls queue back to back, Adam follows them
+        # the views' calls and the optimizer step queue back to back
         with renderer.stream():
-            for i, (v, t) in enumerate(zip(views, targets)):
-                if src is None or tuple(src.shape) != (v.h, v.w):
-                    src = torch.empty((v.h, v.w), dtype=torch.int32, device="cuda")
+            for i, (view, t) in enumerate(zip(views, targets)):
+                if src is None or tuple(src.shape) != (view.h, view.w):
+                    src = torch.empty((view.h, view.w), dtype=torch.int32, device="cuda")
                 # the loop needs neither the image nor its loss gradient
-                renderer.l2_step(planes, sky_scale, v, t, None, src, None, grad, gsky,
+                renderer.l2_step(planes, sky_scale, view, t, None, src, None, grad, gsky,
                                  losses[it:it + 1], accumulate=i > 0, outputs=False)
-        planes.grad = grad
-        opt.step()
-        planes.clamp_(0.0, 1.0)
+            renderer.adam_step(planes, grad, m, v, step_lr, it + 1)
         if on_step is not None:
             on_step(it)
     return losses

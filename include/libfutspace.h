@@ -250,6 +250,18 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
 int fsx_l2_loss(struct futhark_context *ctx, const float *dev_img, const float *dev_target,
                 int64_t n, int32_t accumulate, float *dev_grad, float *dev_loss);
 
+/* One Adam step of a fitting loop (no weight decay, bias-corrected
+ * moments, step >= 1) on dev_param[n] with gradient dev_grad and moments
+ * dev_m, dev_v (zero before the first step), then dev_param clamped to
+ * [lo, hi] -- e.g. the colour planes' [0, 1] -- in one pass over the four
+ * arrays (16-byte aligned device buffers).  The hyper-parameters are
+ * doubles: 1 - beta and the bias corrections are formed in double and
+ * rounded to f32 once, as torch's Adam forms them from Python floats.
+ * Asynchronous on the context stream. */
+int fsx_adam_step(struct futhark_context *ctx, float *dev_param, const float *dev_grad,
+                  float *dev_m, float *dev_v, int64_t n, double lr, double beta1,
+                  double beta2, double eps, int32_t step, float lo, float hi);
+
 /* One fitting step of the differentiable renderer in one call: the same
  * results as fsx_render_f32, then fsx_l2_loss against dev_target[3][h][w]
  * (dev_gimg = 2 (img - target), *dev_loss), then fsx_render_f32_vjp with

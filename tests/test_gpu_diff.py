@@ -462,3 +462,31 @@ def test_fused_step_refuses_frames_beyond_its_partials(ctx):
     torch.cuda.synchronize()
     assert bool((img == 7.0).all()) and bool((gimg == 3.0).all()) and bool((src == 5).all())
     assert bool((gp == 9.0).all()) and float(gs[0]) == 4.0 and float(loss[0]) == 2.0
+
+
+def test_adam_step_matches_torch_adam(ctx):
+    """fsx_adam_step: three steps equal torch's Adam (default betas and
+    eps, no weight decay) within the hardware sqrt / reciprocal's few ulps,
+    then the clamp to [lo, hi]; a length that is not a multiple of 4 takes
+    the scalar tail."""
+    r, _, _, _ = _setup(ctx)
+    rng = np.random.default_rng(31)
+    for n in (4096, 1003):
+        p0 = torch.from_numpy(rng.random(n, dtype=np.float32)).cuda()
+        grads = [torch.from_numpy(rng.standard_normal(n).astype(np.float32)).cuda()
+                 for _ in range(3)]
+        p, m, v = p0.clone(), torch.zeros_like(p0), torch.zeros_like(p0)
+        q = p0.clone().requires_grad_(True)
+        opt = torch.optim.Adam([q], lr=0.01, foreach=False)
+        for t, g in enumerate(grads):
+            r.adam_step(p, g, m, v, 0.01, t + 1, lo=-10.0, hi=10.0)
+            q.grad = g.clone()
+            opt.step()
+        torch.cuda.synchronize()
+        assert float((p - q.detach()).abs().max()) <= 1e-6
+        st = opt.state[q]
+        assert torch.allclose(m, st["exp_avg"], rtol=1e-6, atol=1e-7)
+        assert torch.allclose(v, st["exp_avg_sq"], rtol=1e-6, atol=1e-7)
+        r.adam_step(p, grads[0] * 1e4, m, v, 1.0, 4, lo=0.25, hi=0.75)
+        torch.cuda.synchronize()
+        assert float(p.min()) >= 0.25 and float(p.max()) <= 0.75
