@@ -275,8 +275,8 @@ def bench_diff(ctx, reps):
 def bench_fit(ctx, steps):
     """64 views of 3840x2160 with 4096 slices over the 8192^2 map with
     16-bit heights: colour-map fitting with Adam (lr 0.5 in u8 channel
-    units, planes clamped to [0, 1]), the views' gradients accumulated by
-    the adjoint's scatter-add."""
+    units, planes clamped to [0, 1]: the library's fsx_adam_step), the
+    views' gradients accumulated by the adjoint's scatter-add."""
     import numpy as np
     import torch
 
