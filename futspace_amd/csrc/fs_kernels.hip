@@ -999,21 +999,32 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
 #pragma unroll
         for (int j = 0; j < 4; j++)
             yb[j] = sh.carry[part * 4 + j];
-        for (int32_t r = tid / PARTS; r < H; r += NT / PARTS) {
-            uint32_t px[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                px[j] = painted_value<MODE>(stripe[(part * 4 + j) * Hp + r], r < yb[j], f.sky);
-            uint32_t *dst = out + (size_t)r * (size_t)pitch + col0 + part * 4;
-            if (vec) {
-                *(uint4 *)dst = make_uint4(px[0], px[1], px[2], px[3]);
-            } else {
+        /* the loop is written twice so that the whole-segment one keeps
+         * its single 16-byte store: with the edge test inside one loop the
+         * compiler shares the last element's store between the two forms
+         * and splits the segment into a 12-byte and a 4-byte store */
+        auto rows = [&](auto vec_tag) {
+            constexpr bool VEC = decltype(vec_tag)::value;
+            for (int32_t r = tid / PARTS; r < H; r += NT / PARTS) {
+                uint32_t px[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    if (col0 + part * 4 + j < f.cols)
-                        dst[j] = px[j];
+                    px[j] = painted_value<MODE>(stripe[(part * 4 + j) * Hp + r], r < yb[j], f.sky);
+                uint32_t *dst = out + (size_t)r * (size_t)pitch + col0 + part * 4;
+                if (VEC) {
+                    *(uint4 *)dst = make_uint4(px[0], px[1], px[2], px[3]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (col0 + part * 4 + j < f.cols)
+                            dst[j] = px[j];
+                }
             }
-        }
+        };
+        if (vec)
+            rows(std::true_type{});
+        else
+            rows(std::false_type{});
         return;
     }
 
@@ -1073,26 +1084,34 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         uint32_t p = wfpre[wave * TC + j];
         pre[j] = pre[j] ? pre[j] : p;
     }
-    for (int32_t r = r0; r < r1; r++) {
-        uint32_t px[TC];
+    /* written twice for the same reason as the loop above */
+    auto rows = [&](auto vec_tag) {
+        constexpr bool VEC = decltype(vec_tag)::value;
+        for (int32_t r = r0; r < r1; r++) {
+            uint32_t px[TC];
 #pragma unroll
-        for (int j = 0; j < TC; j++) {
-            uint32_t s = cell(j, r, yb[j]);
-            pre[j] = s ? s : pre[j];
-            px[j] = tile_value<MODE>(pre[j], f.sky);
+            for (int j = 0; j < TC; j++) {
+                uint32_t s = cell(j, r, yb[j]);
+                pre[j] = s ? s : pre[j];
+                px[j] = tile_value<MODE>(pre[j], f.sky);
+            }
+            uint32_t *dst = out + (size_t)r * (size_t)pitch + col0;
+            if (VEC) {
+#pragma unroll
+                for (int j = 0; j < TC; j += 4)
+                    *(uint4 *)(dst + j) = make_uint4(px[j], px[j + 1], px[j + 2], px[j + 3]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < TC; j++)
+                    if (col0 + j < f.cols)
+                        dst[j] = px[j];
+            }
         }
-        uint32_t *dst = out + (size_t)r * (size_t)pitch + col0;
-        if (vec) {
-#pragma unroll
-            for (int j = 0; j < TC; j += 4)
-                *(uint4 *)(dst + j) = make_uint4(px[j], px[j + 1], px[j + 2], px[j + 3]);
-        } else {
-#pragma unroll
-            for (int j = 0; j < TC; j++)
-                if (col0 + j < f.cols)
-                    dst[j] = px[j];
-        }
-    }
+    };
+    if (vec)
+        rows(std::true_type{});
+    else
+        rows(std::false_type{});
 }
 
 template <int TC, int NT, int CMAX, int HBITS, bool HP2, bool CP2, int WPE, int MODE>
