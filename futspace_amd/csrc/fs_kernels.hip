@@ -298,6 +298,22 @@ __device__ __forceinline__ Slice slice_params_tame(const fsk_frame &f, int32_t k
 /* The render kernel                                                   */
 /* ------------------------------------------------------------------ */
 
+/* Inclusive prefix sum over the 64 lanes of a wave, in registers: DPP
+ * row shifts by 1, 2, 4 and 8 scan each row of 16 lanes (lanes shifted in
+ * from outside the row read 0), then the last lane of row 0 is added to
+ * rows 1 and 3 (row_bcast:15; rows 0 and 2 add the old value 0) and lane
+ * 31 to rows 2 and 3 (row_bcast:31). */
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  /* row_shr:1 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  /* row_shr:2 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  /* row_shr:4 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  /* row_shr:8 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); /* row_bcast:15 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); /* row_bcast:31 */
+    return x;
+}
+
 constexpr int kMaxWaves = 16;                    /* 1024-thread workgroups */
 constexpr int kChunk = 2048;                     /* slices per chunk, at most */
 
@@ -487,7 +503,11 @@ __device__ __forceinline__ uint4 color_quad(const fsk_map &m, float x, float y)
 {
     int32_t ix = wrap<CP2>(idx_of(floorf(x)), m.ccols);
     int32_t iy = wrap<CP2>(idx_of(floorf(y)), m.crows);
-    return ((const uint4 *)m.cq)[(size_t)iy * (size_t)m.ccols + (size_t)ix];
+    /* colour maps have at most 65536^2 texels (host-checked), so the quad
+     * index fits 32 bits; the byte offset is formed in 64 */
+    uint32_t t = CP2 ? ((uint32_t)iy << __builtin_ctz((uint32_t)m.ccols)) + (uint32_t)ix
+                     : (uint32_t)iy * (uint32_t)m.ccols + (uint32_t)ix;
+    return ((const uint4 *)m.cq)[t];
 }
 
 /* The colour of a sample at (x, y) from its texel quad q; with qt (the
@@ -867,14 +887,8 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         }
         {
             uint32_t cnt = (uint32_t)__builtin_popcount(vbits);
-            uint32_t incl = cnt;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                uint32_t o = __shfl_up(incl, off);
-                if (lane >= off)
-                    incl += o;
-            }
-            uint32_t total = __shfl(incl, 63);
+            uint32_t incl = wave_incl_sum(cnt);
+            uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             uint32_t b0 = 0;
             if (lane == 0 && total)
                 b0 = atomicAdd(&sh.qn, total);
