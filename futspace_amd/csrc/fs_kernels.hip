@@ -1932,33 +1932,51 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
         a.img[ch * fplane + p] = px[ch];
 }
 
-/* Segmented suffix sum over the lanes of a wave: lanes holding the same
- * key in a contiguous run form a segment, and each segment's first lane
- * ends up with the segment's total of v[0..2].  Returns whether this
- * lane heads its segment. */
+/* One step of segment_sum3: each lane takes (v, f) of the lane CTRL names
+ * (DPP) unless its accumulated range already holds its segment's first
+ * lane (f).  Lanes with no source (shifted out of their row, or in rows
+ * that ROWS leaves out) take (0, 0), which changes nothing. */
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void segment_step(float v[3], int32_t &f)
+{
+    constexpr bool kShift = CTRL < 0x140;      /* row shifts read 0 outside the row */
+    float d[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        d[c] = __int_as_float(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(v[c]), CTRL, ROWS, 0xf, kShift));
+    const int32_t fd = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, kShift);
+    if (!f) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            v[c] += d[c];
+        f = fd;
+    }
+}
+
+/* Segmented inclusive prefix sum over the lanes of a wave, in registers:
+ * lanes holding the same key in a contiguous run form a segment, and each
+ * lane ends up with the sum of v[0..2] from its segment's first lane to
+ * itself, so the segment's last lane holds the segment's total.  Steps of
+ * 1, 2, 4 and 8 lanes scan each row of 16 lanes (DPP row shifts); then
+ * the last lane of rows 0 and 2 is carried into rows 1 and 3
+ * (row_bcast:15) and lane 31 into rows 2 and 3 (row_bcast:31).  Returns
+ * whether this lane ends its segment. */
 __device__ __forceinline__ bool segment_sum3(uint32_t key, float v[3])
 {
-    const int32_t lane = threadIdx.x & 63;
-    uint32_t next = __shfl_down(key, 1);
-    uint32_t prev = __shfl_up(key, 1);
-    int32_t tail = (lane == 63 || next != key) ? 1 : 0;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        float o0 = __shfl_down(v[0], off), o1 = __shfl_down(v[1], off);
-        float o2 = __shfl_down(v[2], off);
-        int32_t to = __shfl_down(tail, off);
-        if (lane + off > 63) {
-            o0 = o1 = o2 = 0.0f;
-            to = 1;
-        }
-        if (!tail) {
-            v[0] += o0;
-            v[1] += o1;
-            v[2] += o2;
-        }
-        tail |= to;
-    }
-    return lane == 0 || prev != key;
+    const int32_t k = (int32_t)key;
+    /* the keys of lane - 1 and lane + 1 (wave shifts; the first and the
+     * last lane see a key that differs from their own) */
+    const int32_t prev = __builtin_amdgcn_update_dpp(~k, k, 0x138, 0xf, 0xf, false);
+    const int32_t next = __builtin_amdgcn_update_dpp(~k, k, 0x130, 0xf, 0xf, false);
+    int32_t f = prev != k ? 1 : 0;
+    segment_step<0x111, 0xf>(v, f);            /* row_shr:1 */
+    segment_step<0x112, 0xf>(v, f);            /* row_shr:2 */
+    segment_step<0x114, 0xf>(v, f);            /* row_shr:4 */
+    segment_step<0x118, 0xf>(v, f);            /* row_shr:8 */
+    segment_step<0x142, 0xa>(v, f);            /* row_bcast:15 */
+    segment_step<0x143, 0xc>(v, f);            /* row_bcast:31 */
+    return next != k;
 }
 
 constexpr int kGradThreads = 512;
@@ -2182,9 +2200,9 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const bool head = segment_sum3(key[t], val[t]);
+                const bool last = segment_sum3(key[t], val[t]);
                 bool inserted = false;
-                if (head && key[t] != kNoKey)
+                if (last && key[t] != kNoKey)
                     inserted = table_add(keys, vals, &fill, a.gcp, plane, key[t], val[t]);
                 /* the slot count is kept per wave, not per insertion: one
                  * LDS atomic instead of one per inserting lane */
