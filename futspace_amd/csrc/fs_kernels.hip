@@ -314,6 +314,34 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
     return x;
 }
 
+/* Per-column minima over the slice groups of a wave of an 8-column tile
+ * (lane = 8 * group + column, rows at most H): excl = the minimum over
+ * the groups before the lane's (H for the first group), total = over all
+ * eight.  In registers: DPP folds the two groups of each 16-lane row
+ * (row_shr:8, then row_ror:8 to give the row's minimum to all its lanes),
+ * and the permlane swaps hand every row the minima of all four rows with
+ * the columns aligned (permlane32_swap exchanges rows 2-3 of its first
+ * operand with rows 0-1 of its second; permlane16_swap the odd rows of the
+ * first with the even rows of the second). */
+__device__ __forceinline__ void wave_column_scan8(int32_t x, int32_t H, int32_t lane,
+                                                  int32_t &excl, int32_t &total)
+{
+    /* the even group of the row in the odd group's lanes (row_shr:8; the
+     * even group's lanes keep H) */
+    const int32_t e = __builtin_amdgcn_update_dpp(H, x, 0x118, 0xf, 0xf, false);
+    const int32_t i = min(x, e);
+    const int32_t t = min(i, __builtin_amdgcn_update_dpp(H, i, 0x128, 0xf, 0xf, false));
+    const auto y = __builtin_amdgcn_permlane32_swap(t, t, false, false);   /* T0 T1 T0 T1 | T2 T3 T2 T3 */
+    const auto a = __builtin_amdgcn_permlane16_swap(y[0], y[0], false, false);   /* T0 | T1 */
+    const auto b = __builtin_amdgcn_permlane16_swap(y[1], y[1], false, false);   /* T2 | T3 */
+    const int32_t t0 = (int32_t)a[0], m01 = min(t0, (int32_t)a[1]);
+    const int32_t m012 = min(m01, (int32_t)b[0]);
+    total = min(m012, (int32_t)b[1]);
+    const int32_t row = lane >> 4;
+    const int32_t before = row == 0 ? H : row == 1 ? t0 : row == 2 ? m01 : m012;
+    excl = (lane & 8) ? min(before, e) : before;
+}
+
 constexpr int kMaxWaves = 16;                    /* 1024-thread workgroups */
 constexpr int kChunk = 2048;                     /* slices per chunk, at most */
 
@@ -844,18 +872,26 @@ __device__ __forceinline__ void render_tile(const fsk_map &m, const fsk_frame &f
         }
 
         /* phase 2 */
-        int32_t v = lmin;
+        int32_t excl;
+        if constexpr (TC == 8) {
+            int32_t total;
+            wave_column_scan8(lmin, H, lane, excl, total);
+            if (lane >= 64 - TC)
+                sh.wtot[wave][c] = total;
+        } else {
+            int32_t v = lmin;
 #pragma unroll
-        for (int off = TC; off < 64; off <<= 1) {
-            int32_t o = __shfl_up(v, off);
-            if (lane >= off)
-                v = min(v, o);
+            for (int off = TC; off < 64; off <<= 1) {
+                int32_t o = __shfl_up(v, off);
+                if (lane >= off)
+                    v = min(v, o);
+            }
+            excl = __shfl_up(v, TC);
+            if (lane < TC)
+                excl = H;
+            if (lane >= 64 - TC)
+                sh.wtot[wave][c] = v;
         }
-        int32_t excl = __shfl_up(v, TC);
-        if (lane < TC)
-            excl = H;
-        if (lane >= 64 - TC)
-            sh.wtot[wave][c] = v;
         /* the chunk's carry-in is read before the barrier: the last slice
          * group overwrites it in phase 3 without a barrier in between (the
          * first chunk's is H) */
