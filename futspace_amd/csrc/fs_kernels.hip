@@ -1834,17 +1834,10 @@ struct DiffArgs {
     float *sky_part;       /* fused step: per-workgroup sky terms */
 };
 
-/* The relaxed argb.mix channel: sqrt(m1' a^2 + m2' b^2) with the weights
- * normalised by m1 + m2 -- a division that is exact and skipped when the
- * sum is exactly 1 (bilinear weights at |x| >= 1). */
-__device__ __forceinline__ float mixf(float m1, float a, float m2, float b)
+/* The relaxed argb.mix channel: sqrt(m1' a^2 + m2' b^2) with weights
+ * m1', m2' already normalised by norm_weights. */
+__device__ __forceinline__ float mixn(float m1n, float a, float m2n, float b)
 {
-    float m12 = m1 + m2;
-    float m1n = m1, m2n = m2;
-    if (m12 != 1.0f) {
-        m1n = m1 / m12;
-        m2n = m2 / m12;
-    }
     return sqrt_rn((m1n * (a * a)) + (m2n * (b * b)));
 }
 
@@ -1855,6 +1848,35 @@ struct Bilin {
     uint32_t t[4];
     float wx0, wx1, wy0, wy1;
 };
+
+/* argb.mix's weights divided by their sum, for the x pair and the y pair
+ * of a bilinear sample (w = wx0', wx1', wy0', wy1').  A pair sums to
+ * exactly 1 unless its coordinate is an integer (sum 0) or within 1 of the
+ * map origin, where the division is an exact no-op when skipped, so the
+ * divisions run on a wave-uniform branch only when some lane of the wave
+ * needs them (the empty asm keeps the compiler from evaluating them on
+ * every wave). */
+__device__ __forceinline__ void norm_weights(const Bilin &b, float w[4])
+{
+    w[0] = b.wx0;
+    w[1] = b.wx1;
+    w[2] = b.wy0;
+    w[3] = b.wy1;
+    const float sx = b.wx0 + b.wx1, sy = b.wy0 + b.wy1;
+    const bool dx = sx != 1.0f, dy = sy != 1.0f;
+    if (__ballot(dx || dy)) {
+        float x0 = b.wx0, x1 = b.wx1, y0 = b.wy0, y1 = b.wy1;
+        asm volatile("" : "+v"(x0), "+v"(x1), "+v"(y0), "+v"(y1));
+        if (dx) {
+            w[0] = x0 / sx;
+            w[1] = x1 / sx;
+        }
+        if (dy) {
+            w[2] = y0 / sy;
+            w[3] = y1 / sy;
+        }
+    }
+}
 
 /* Texel offsets are 32-bit: a plane holds at most 2^31 texels (checked by
  * the host).  P2: both plane sides are powers of two, so the floor-mod is
@@ -1911,14 +1933,16 @@ __device__ __forceinline__ void sample_gather(const DiffArgs &a, const Bilin &b,
 __device__ __forceinline__ void sample_mix(const Bilin &b, const float cv[3][4],
                                            const float sf[4], float sv[3][4], float px[3])
 {
+    float w[4];
+    norm_weights(b, w);
 #pragma unroll
     for (int ch = 0; ch < 3; ch++) {
 #pragma unroll
         for (int t = 0; t < 4; t++)
             sv[ch][t] = cv[ch][t] * sf[t];
-        float X1 = mixf(b.wx0, sv[ch][0], b.wx1, sv[ch][1]);
-        float X2 = mixf(b.wx0, sv[ch][2], b.wx1, sv[ch][3]);
-        px[ch] = mixf(b.wy0, X1, b.wy1, X2);
+        float X1 = mixn(w[0], sv[ch][0], w[1], sv[ch][1]);
+        float X2 = mixn(w[0], sv[ch][2], w[1], sv[ch][3]);
+        px[ch] = mixn(w[2], X1, w[3], X2);
     }
 }
 
@@ -1947,10 +1971,11 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
             px[ch] = sky_base(ch) * a.sky_scale;
     } else {
         const Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
-        float sf[4];
+        float sf[4], w[4];
 #pragma unroll
         for (int t = 0; t < 4; t++)
             sf[t] = a.sf[b.t[t]];
+        norm_weights(b, w);
 #pragma unroll
         for (int ch = 0; ch < 3; ch++) {
             const float *P = a.cp + ch * plane;
@@ -1958,9 +1983,9 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
 #pragma unroll
             for (int t = 0; t < 4; t++)
                 sv[t] = P[b.t[t]] * sf[t];
-            float X1 = mixf(b.wx0, sv[0], b.wx1, sv[1]);
-            float X2 = mixf(b.wx0, sv[2], b.wx1, sv[3]);
-            px[ch] = mixf(b.wy0, X1, b.wy1, X2);
+            float X1 = mixn(w[0], sv[0], w[1], sv[1]);
+            float X2 = mixn(w[0], sv[2], w[1], sv[3]);
+            px[ch] = mixn(w[2], X1, w[3], X2);
         }
     }
 #pragma unroll
@@ -2086,17 +2111,9 @@ __device__ __forceinline__ void sample_grad(const DiffArgs &a, const SampleEval 
     const Bilin &b = e.b;
     const float(&sv)[3][4] = e.sv;
     const float(&px)[3] = e.px;
-    const float wxs = b.wx0 + b.wx1, wys = b.wy0 + b.wy1;
-    float wx0n = b.wx0, wx1n = b.wx1, wy0n = b.wy0, wy1n = b.wy1;
-    if (wxs != 1.0f) {            /* exact and skipped when the sum is 1 */
-        wx0n = b.wx0 / wxs;
-        wx1n = b.wx1 / wxs;
-    }
-    if (wys != 1.0f) {
-        wy0n = b.wy0 / wys;
-        wy1n = b.wy1 / wys;
-    }
-    const float wt[4] = {wy0n * wx0n, wy0n * wx1n, wy1n * wx0n, wy1n * wx1n};
+    float w[4];
+    norm_weights(b, w);
+    const float wt[4] = {w[2] * w[0], w[2] * w[1], w[3] * w[0], w[3] * w[1]};
     float q[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ch++)
