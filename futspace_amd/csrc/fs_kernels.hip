@@ -1993,51 +1993,77 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
         a.img[ch * fplane + p] = px[ch];
 }
 
-/* One step of segment_sum3: each lane takes (v, f) of the lane CTRL names
- * (DPP) unless its accumulated range already holds its segment's first
- * lane (f).  Lanes with no source (shifted out of their row, or in rows
- * that ROWS leaves out) take (0, 0), which changes nothing. */
-template <int CTRL, int ROWS>
-__device__ __forceinline__ void segment_step(float v[3], int32_t &f)
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;     /* an empty texel slot, a lane with no sample */
+
+/* One step of segment_sum_quad: each lane takes (v, f) of the lane CTRL
+ * names (DPP) unless its accumulated range already holds its segment's
+ * first lane (f).  Lanes with no source (shifted out of their row, or in
+ * rows that ROWS leaves out) take (0, 0), which changes nothing. */
+template <int CTRL, int ROWS, int N>
+__device__ __forceinline__ void segment_step(float (&v)[N], int32_t &f)
 {
     constexpr bool kShift = CTRL < 0x140;      /* row shifts read 0 outside the row */
-    float d[3];
+    float d[N];
 #pragma unroll
-    for (int c = 0; c < 3; c++)
+    for (int c = 0; c < N; c++)
         d[c] = __int_as_float(
             __builtin_amdgcn_update_dpp(0, __float_as_int(v[c]), CTRL, ROWS, 0xf, kShift));
     const int32_t fd = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, kShift);
     if (!f) {
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+        for (int c = 0; c < N; c++)
             v[c] += d[c];
         f = fd;
     }
 }
 
-/* Segmented inclusive prefix sum over the lanes of a wave, in registers:
- * lanes holding the same key in a contiguous run form a segment, and each
- * lane ends up with the sum of v[0..2] from its segment's first lane to
- * itself, so the segment's last lane holds the segment's total.  Steps of
- * 1, 2, 4 and 8 lanes scan each row of 16 lanes (DPP row shifts); then
- * the last lane of rows 0 and 2 is carried into rows 1 and 3
- * (row_bcast:15) and lane 31 into rows 2 and 3 (row_bcast:31).  Returns
- * whether this lane ends its segment. */
-__device__ __forceinline__ bool segment_sum3(uint32_t key, float v[3])
+/* Segmented inclusive prefix sum of a sample's four texel values over the
+ * lanes of a wave, in registers: lanes holding the same texel quad (all
+ * four keys equal, none of them kNoKey) in a contiguous run form a
+ * segment, and each lane ends up with the sums of val from its segment's
+ * first lane to itself, so the segment's last lane holds the segment's
+ * totals.  Steps of 1, 2, 4 and 8 lanes scan each row of 16 lanes (DPP
+ * row shifts); then the last lane of rows 0 and 2 is carried into rows 1
+ * and 3 (row_bcast:15) and lane 31 into rows 2 and 3 (row_bcast:31).  A
+ * wave in which no lane shares its quad with the lane before skips the
+ * scan (every lane is a segment of its own).  Returns whether this lane
+ * ends its segment. */
+__device__ __forceinline__ bool segment_sum_quad(const uint32_t key[4], float (&val)[4][3])
 {
-    const int32_t k = (int32_t)key;
     /* the keys of lane - 1 and lane + 1 (wave shifts; the first and the
-     * last lane see a key that differs from their own) */
-    const int32_t prev = __builtin_amdgcn_update_dpp(~k, k, 0x138, 0xf, 0xf, false);
-    const int32_t next = __builtin_amdgcn_update_dpp(~k, k, 0x130, 0xf, 0xf, false);
-    int32_t f = prev != k ? 1 : 0;
+     * last lane see a key that differs from their own).  Every lane reads
+     * both neighbours: a DPP read from a lane that a branch has switched
+     * off does not see that lane's key, so the comparisons are combined
+     * without short-circuiting */
+    uint32_t diff_prev = key[0] == kNoKey ? 1u : 0u, diff_next = diff_prev;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int32_t k = (int32_t)key[t];
+        diff_prev |= (uint32_t)(__builtin_amdgcn_update_dpp(~k, k, 0x138, 0xf, 0xf, false) ^ k);
+        diff_next |= (uint32_t)(__builtin_amdgcn_update_dpp(~k, k, 0x130, 0xf, 0xf, false) ^ k);
+    }
+    const bool same_prev = diff_prev == 0u, same_next = diff_next == 0u;
+    if (!__ballot(same_prev))
+        return true;
+    float v[12];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            v[t * 3 + c] = val[t][c];
+    int32_t f = same_prev ? 0 : 1;
     segment_step<0x111, 0xf>(v, f);            /* row_shr:1 */
     segment_step<0x112, 0xf>(v, f);            /* row_shr:2 */
     segment_step<0x114, 0xf>(v, f);            /* row_shr:4 */
     segment_step<0x118, 0xf>(v, f);            /* row_shr:8 */
     segment_step<0x142, 0xa>(v, f);            /* row_bcast:15 */
     segment_step<0x143, 0xc>(v, f);            /* row_bcast:31 */
-    return next != k;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            val[t][c] = v[t * 3 + c];
+    return !same_next;
 }
 
 constexpr int kGradThreads = 512;
@@ -2046,7 +2072,6 @@ constexpr int kGradBlockRows = (kGradThreads / 64) * kGradRows;   /* 16 */
 constexpr int kGradCapLog = 12;
 constexpr int kGradCap = 1 << kGradCapLog;   /* texel slots of a workgroup's LDS table */
 constexpr int kGradProbes = 2;
-constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
 /* Add v[0..2] to texel `key` of the workgroup's LDS table (open addressing,
  * linear probing); returns whether this call took a new slot.  A texel that
@@ -2251,9 +2276,9 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
                 else
                     sample_terms<P2>(a, k, col, plane, run, key, val);
             }
+            const bool last = segment_sum_quad(key, val);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const bool last = segment_sum3(key[t], val[t]);
                 bool inserted = false;
                 if (last && key[t] != kNoKey)
                     inserted = table_add(keys, vals, &fill, a.gcp, plane, key[t], val[t]);
