@@ -1579,6 +1579,9 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
     s.cmax = kChunk;
     int chunk = (t.chunk >= 256 && t.chunk <= kChunk) ? t.chunk : 0;
     int c8 = chunk && chunk <= 512 ? chunk : 512;
+    auto fits8 = [&](int c) {
+        return render_lds_bytes(8, 1024, rows, c) + kStaticLds <= lds_per_cu();
+    };
     if (t.tc != 4 && t.wg_per_cu == 2 &&
         2 * (render_lds_bytes(8, 512, rows, c8) + kStaticLds) <= lds_per_cu()) {
         s.tc = 8;
@@ -1586,12 +1589,14 @@ inline Shape resolve_shape(int32_t rows, const fsk_tuning &t)
         s.wg = 2;
         s.chunk = c8;
         s.cmax = 512;
-    } else if (t.tc != 4 && render_lds_bytes(8, 1024, rows, chunk ? chunk : kChunk) + kStaticLds <=
-                                lds_per_cu()) {
+    } else if (t.tc != 4 && (chunk ? fits8(chunk) : fits8(kChunk) || fits8(kChunk / 2))) {
+        /* left automatic, a frame too tall for 8-column tiles of the whole
+         * chunk keeps them with half the chunk: measured faster than
+         * 4-column tiles (2160 rows, 4096 slices: 49 against 55 us) */
         s.tc = 8;
         s.nt = 1024;
         s.wg = 1;
-        s.chunk = chunk ? chunk : kChunk;
+        s.chunk = chunk ? chunk : fits8(kChunk) ? kChunk : kChunk / 2;
     } else {
         s.tc = 4;
         int c2 = chunk ? chunk : 1024;
