@@ -2144,17 +2144,20 @@ __device__ __forceinline__ void sample_grad(const DiffArgs &a, const SampleEval 
     float w[4];
     norm_weights(b, w);
     const float wt[4] = {w[2] * w[0], w[2] * w[1], w[3] * w[0], w[3] * w[1]};
+    /* gradients are not bit-exact quantities: the hardware reciprocal
+     * (within an ulp) stands in for the division, and the texel's weight
+     * and shade factor are multiplied once */
     float q[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ch++)
-        q[ch] = px[ch] > 0.0f ? run[ch] / px[ch] : 0.0f;
+        q[ch] = px[ch] > 0.0f ? run[ch] * __builtin_amdgcn_rcpf(px[ch]) : 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const float sf = e.sf[t];
+        const float ws = wt[t] * e.sf[t];
         key[t] = b.t[t];
 #pragma unroll
         for (int ch = 0; ch < 3; ch++)
-            val[t][ch] = ((q[ch] * wt[t]) * sv[ch][t]) * sf;
+            val[t][ch] = (q[ch] * ws) * sv[ch][t];
     }
 }
 
