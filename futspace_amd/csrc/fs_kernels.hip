@@ -1852,7 +1852,33 @@ __device__ __forceinline__ float mixn(float m1n, float a, float m2n, float b)
 struct Bilin {
     uint32_t t[4];
     float wx0, wx1, wy0, wy1;
+    bool pair;               /* fx is not the map's last column */
 };
+
+/* The four texels of a bilinear quad from one plane.  When no lane of
+ * the wave has its quad wrap round the map's last column, texels t1 and
+ * t3 follow t0 and t2 in memory, and each pair is one 8-byte load (4-byte
+ * aligned, which global loads allow).  At an integer x the quad's second
+ * texel is its first (cx = fx) and the pair loads the next one instead;
+ * the x weights are then 0 and their sum 0, so the colour is NaN whatever
+ * the texel holds. */
+typedef float float2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+__device__ __forceinline__ void quad_load(const float *P, const Bilin &b, bool pairs,
+                                          float v[4])
+{
+    if (pairs) {
+        const float2_a4 lo = *(const float2_a4 *)(P + b.t[0]);
+        const float2_a4 hi = *(const float2_a4 *)(P + b.t[2]);
+        v[0] = lo.x;
+        v[1] = lo.y;
+        v[2] = hi.x;
+        v[3] = hi.y;
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            v[t] = P[b.t[t]];
+    }
+}
 
 /* argb.mix's weights divided by their sum, for the x pair and the y pair
  * of a bilinear sample (w = wx0', wx1', wy0', wy1').  A pair sums to
@@ -1906,6 +1932,7 @@ __device__ __forceinline__ Bilin bilin_of(const fsk_frame &f, int32_t k, int32_t
     b.wx1 = x - fx;
     b.wy0 = cy - y;
     b.wy1 = y - fy;
+    b.pair = ifx < wc - 1;
     return b;
 }
 
@@ -1921,6 +1948,8 @@ __device__ __forceinline__ float sky_base(int ch)
 __device__ __forceinline__ void sample_gather(const DiffArgs &a, const Bilin &b, size_t plane,
                                               float cv[3][4], float sf[4])
 {
+    /* one load per texel: the paired loads of quad_load measured slower
+     * in the adjoint (its gathers sit under a per-lane branch) */
 #pragma unroll
     for (int t = 0; t < 4; t++)
         sf[t] = a.sf[b.t[t]];
@@ -1977,17 +2006,16 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
     } else {
         const Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
         float sf[4], w[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-            sf[t] = a.sf[b.t[t]];
+        const bool pairs = __ballot(!b.pair) == 0;
+        quad_load(a.sf, b, pairs, sf);
         norm_weights(b, w);
 #pragma unroll
         for (int ch = 0; ch < 3; ch++) {
-            const float *P = a.cp + ch * plane;
             float sv[4];
+            quad_load(a.cp + ch * plane, b, pairs, sv);
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                sv[t] = P[b.t[t]] * sf[t];
+                sv[t] *= sf[t];
             float X1 = mixn(w[0], sv[0], w[1], sv[1]);
             float X2 = mixn(w[0], sv[2], w[1], sv[3]);
             px[ch] = mixn(w[2], X1, w[3], X2);
