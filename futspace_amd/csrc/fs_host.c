@@ -61,14 +61,15 @@ struct futhark_context {
 };
 
 #define FS_PARTIALS 32768     /* floats in d_partials */
-#define FS_STEP_BLOCK_COLS 64 /* pixel blocks of the adjoint (and of the   */
-#define FS_STEP_BLOCK_ROWS 16 /* fused step): one partial sum per block    */
+#define FS_STEP_BLOCK_COLS 64 /* pixel blocks of the adjoint (and of the fused step) */
 
-/* Pixel blocks of a w x h frame in the differentiable passes. */
+/* Pixel blocks of a w x h frame in the differentiable passes (64 columns
+ * by fsk_grad_block_rows rows): one partial sum per block. */
 static int64_t step_blocks(int32_t w, int32_t h)
 {
+    const int32_t br = fsk_grad_block_rows(w, h);
     return (int64_t)((w + FS_STEP_BLOCK_COLS - 1) / FS_STEP_BLOCK_COLS) *
-           (int64_t)((h + FS_STEP_BLOCK_ROWS - 1) / FS_STEP_BLOCK_ROWS);
+           (int64_t)((h + br - 1) / br);
 }
 
 #define FS_PROF_MAX 4096
@@ -1678,7 +1679,7 @@ int fsx_render_f32_vjp(struct futhark_context *ctx, const struct fsx_view *view,
         set_error(ctx, "fsx_render_f32_vjp: a %d x %d frame has %lld blocks of %d x %d pixels, "
                   "more than the %d partial sums of the adjoint", view->w, view->h,
                   (long long)step_blocks(view->w, view->h), FS_STEP_BLOCK_COLS,
-                  FS_STEP_BLOCK_ROWS, FS_PARTIALS);
+                  fsk_grad_block_rows(view->w, view->h), FS_PARTIALS);
         return 1;
     }
     float *part;
@@ -1723,7 +1724,7 @@ static int l2_step(struct futhark_context *ctx, const char *what, int noalloc,
         set_error(ctx, "%s: a %d x %d frame has %lld blocks of %d x %d pixels, more than the "
                   "%d loss partial sums of the fused step", what, view->w, view->h,
                   (long long)step_blocks(view->w, view->h), FS_STEP_BLOCK_COLS,
-                  FS_STEP_BLOCK_ROWS, FS_PARTIALS / 2);
+                  fsk_grad_block_rows(view->w, view->h), FS_PARTIALS / 2);
         return 1;
     }
     float *part = NULL;

@@ -81,6 +81,9 @@ int fsk_render(const fsk_map *m, const fsk_frame *f, const fsk_tuning *t, uint32
  * slower one-thread-per-column kernel), and the rows of a batched view. */
 int32_t fsk_render_max_rows(void);
 int32_t fsk_batch_max_rows(void);
+/* Pixel rows of one adjoint workgroup's 64-column block in a w x h frame
+ * (16, or 32 in frames of 2^22 pixels or more); one partial sum per block. */
+int32_t fsk_grad_block_rows(int32_t w, int32_t h);
 
 /* One view of a batched render, as the persistent kernel reads it. */
 typedef struct {

@@ -228,10 +228,10 @@ int fsx_render_stateless(struct futhark_context *ctx, const int32_t *dev_color,
  * dL/d sky_scale into *dev_gsky, given dev_gimg = dL/d img and the src of
  * the forward pass; accumulate = 0 clears both first.  f32 atomics: the
  * summation order, hence the last bits, varies from run to run.  Its
- * per-workgroup partial sums limit the frame to ceil(w/64) * ceil(h/16)
- * <= 32768 blocks of 64 x 16 pixels (h <= 4096 rows: every w up to 8192;
- * h <= 7680: w up to 4352; taller frames, fewer columns);
- * larger frames are refused before anything is enqueued.
+ * per-workgroup partial sums limit the frame to ceil(w/64) * ceil(h/b)
+ * <= 32768 blocks, b = 16 rows, or 32 in frames of 2^22 pixels or more
+ * (every frame up to 8192 x 8192); larger frames are refused before
+ * anything is enqueued.
  *
  * All three are asynchronous on the context stream. */
 int fsx_map_shade_factor(struct futhark_context *ctx, const struct fsx_map *map, float *dev_sf);
@@ -275,10 +275,11 @@ int fsx_adam_step(struct futhark_context *ctx, float *dev_param, const float *de
  * one-workgroup sum of the loss and sky partials.  The loss is summed per
  * workgroup in a fixed order, so the loss of a step is reproducible; the
  * gradients carry the f32-atomic summation order of fsx_render_f32_vjp.
- * The loss and sky partial sums (one each per 64 x 16 pixel block) limit
- * the frame to ceil(w/64) * ceil(h/16) <= 16384 blocks (4096 x 4096, or
- * 8192 columns by 2048 rows); larger frames are refused before anything is
- * enqueued.  Asynchronous on the context stream. */
+ * The loss and sky partial sums (one each per block of 64 columns by
+ * b = 16 rows, or 32 in frames of 2^22 pixels or more) limit the frame to
+ * ceil(w/64) * ceil(h/b) <= 16384 blocks (every frame up to 8192 x 4096);
+ * larger frames are refused before anything is enqueued.  Asynchronous
+ * on the context stream. */
 int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *map,
                            const struct fsx_view *view, const float *dev_planes,
                            const float *dev_sf, int32_t hc, int32_t wc, float sky_scale,
@@ -289,7 +290,7 @@ int fsx_render_f32_l2_step(struct futhark_context *ctx, const struct fsx_map *ma
 /* fsx_render_f32_l2_step that allocates nothing: no counterpart in the
  * reference; it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving or training loops).  Same
- * arguments, results and frame limit (16384 blocks of 64 x 16 pixels);
+ * arguments, results and frame limit (16384 blocks, 8192 x 4096);
  * the per-workgroup partial sums live in a fixed scratch array of 32768
  * floats (128 KiB) per device, set up when the first context on the
  * device is created and held until the process ends, instead of the

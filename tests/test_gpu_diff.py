@@ -419,10 +419,10 @@ def test_fused_step_gradient_clear(ctx, w):
 
 
 def test_fused_step_of_a_4096x2160_frame(ctx):
-    """A 4096 x 2160 frame has 8640 blocks of 64 x 16 pixels, more than a
-    quarter of the partial-sum scratch: the fused step still equals the
-    three separate calls (image, src and loss gradient bit for bit, loss
-    within f32 summation order, gradients within 1e-5)."""
+    """A 4096 x 2160 frame (2^22 pixels or more: the adjoint's blocks are 64
+    x 32 pixels, 4352 of them): the fused step still equals the three
+    separate calls (image, src and loss gradient bit for bit, loss within
+    f32 summation order, gradients within 1e-5)."""
     r, col, _, _ = _setup(ctx, seed=11, size=256)
     v = View(x=120.3, y=77.7, height=280.0, angle=0.9, horizon=700.0, distance=400.0, fov=1.2,
              h=2160, w=4096, d=0.001, z0=0.4, n_slices=900)
@@ -442,13 +442,13 @@ def test_fused_step_of_a_4096x2160_frame(ctx):
 
 
 def test_fused_step_refuses_frames_beyond_its_partials(ctx):
-    """8192 x 2112 needs 16896 loss partials (more than 16384): the step is
-    refused with a message naming the limit, before anything is enqueued
-    (the outputs keep their contents)."""
+    """8192 x 4128 needs 16512 loss partials (blocks of 64 x 32 pixels, more
+    than 16384): the step is refused with a message naming the limit,
+    before anything is enqueued (the outputs keep their contents)."""
     from futspace_amd.renderer import FutspaceError
     r, col, _, _ = _setup(ctx, seed=11, size=128)
     v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=60.0, distance=300.0, fov=1.2,
-             h=2112, w=8192, d=0.01, z0=0.5)
+             h=4128, w=8192, d=0.01, z0=0.5)
     planes = torch.from_numpy(planes_of(col)).cuda()
     target = torch.zeros((3, v.h, v.w), dtype=torch.float32, device="cuda")
     img = torch.full_like(target, 7.0)

@@ -71,7 +71,7 @@ def test_noalloc_step_refuses_oversized_frames(ctx):
     r = DiffRenderer(ctx, m)
     planes = torch.from_numpy(planes_of(col)).cuda()
     v = View(x=64.3, y=100.7, height=260.0, angle=0.4, horizon=60.0, distance=300.0, fov=1.2,
-             h=2112, w=8192, d=0.01, z0=0.5)
+             h=4128, w=8192, d=0.01, z0=0.5)
     target = torch.zeros((3, v.h, v.w), dtype=torch.float32, device="cuda")
     with pytest.raises(Exception, match="16384"):
         _step(ctx, r, "fsx_render_f32_l2_step_noalloc", planes, v, target, 0.37)
