@@ -1912,22 +1912,29 @@ __device__ __forceinline__ void norm_weights(const Bilin &b, float w[4])
 /* Texel offsets are 32-bit: a plane holds at most 2^31 texels (checked by
  * the host).  P2: both plane sides are powers of two, so the floor-mod is
  * a mask (also for the INT_MIN of an out-of-range cast). */
-template <bool P2>
+template <bool P2, bool TAME = false>
 __device__ __forceinline__ Bilin bilin_of(const fsk_frame &f, int32_t k, int32_t col, int32_t hc,
                                           int32_t wc)
 {
-    Slice sl = f.tame ? slice_params_tame(f, k) : slice_params(f, k);
+    Slice sl = TAME || f.tame ? slice_params_tame(f, k) : slice_params(f, k);
     float fcol = (float)col;
     float x = sl.sx + (fcol * sl.dx);
     float y = sl.sy + (fcol * sl.dy);
     float fx = floorf(x), cx = ceilf(x), fy = floorf(y), cy = ceilf(y);
-    int32_t ify = wrap<P2>(i32_f32_x86(fy), hc), icy = wrap<P2>(i32_f32_x86(cy), hc);
-    int32_t ifx = wrap<P2>(i32_f32_x86(fx), wc), icx = wrap<P2>(i32_f32_x86(cx), wc);
+    /* TAME: a tame frame keeps |x|, |y| < 2^22, where the x86 conversion
+     * is the plain one */
+    auto cvt = [](float v) { return TAME ? (int32_t)v : i32_f32_x86(v); };
+    int32_t ify = wrap<P2>(cvt(fy), hc), icy = wrap<P2>(cvt(cy), hc);
+    int32_t ifx = wrap<P2>(cvt(fx), wc), icx = wrap<P2>(cvt(cx), wc);
+    const uint32_t ry = P2 ? (uint32_t)ify << __builtin_ctz((uint32_t)wc)
+                           : (uint32_t)ify * (uint32_t)wc;
+    const uint32_t rc = P2 ? (uint32_t)icy << __builtin_ctz((uint32_t)wc)
+                           : (uint32_t)icy * (uint32_t)wc;
     Bilin b;
-    b.t[0] = (uint32_t)ify * (uint32_t)wc + (uint32_t)ifx;
-    b.t[1] = (uint32_t)ify * (uint32_t)wc + (uint32_t)icx;
-    b.t[2] = (uint32_t)icy * (uint32_t)wc + (uint32_t)ifx;
-    b.t[3] = (uint32_t)icy * (uint32_t)wc + (uint32_t)icx;
+    b.t[0] = ry + (uint32_t)ifx;
+    b.t[1] = ry + (uint32_t)icx;
+    b.t[2] = rc + (uint32_t)ifx;
+    b.t[3] = rc + (uint32_t)icx;
     b.wx0 = cx - x;
     b.wx1 = x - fx;
     b.wy0 = cy - y;
@@ -1987,7 +1994,7 @@ __device__ __forceinline__ void sample_mix(const Bilin &b, const float cv[3][4],
  * sixteen gathers first (which needs more registers per wave), and so did
  * one pixel per thread against two or four, and against listing each
  * column's runs to colour each visible sample once. */
-template <bool P2>
+template <bool P2, bool TAME>
 __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
 {
     const int32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -2004,7 +2011,7 @@ __global__ __launch_bounds__(256) void k_diff_color(DiffArgs a)
         for (int ch = 0; ch < 3; ch++)
             px[ch] = sky_base(ch) * a.sky_scale;
     } else {
-        const Bilin b = bilin_of<P2>(a.f, k, col, a.hc, a.wc);
+        const Bilin b = bilin_of<P2, TAME>(a.f, k, col, a.hc, a.wc);
         float sf[4], w[4];
         const bool pairs = __ballot(!b.pair) == 0;
         quad_load(a.sf, b, pairs, sf);
@@ -2738,10 +2745,10 @@ int fsk_diff_color(const fsk_frame *f, const float *d_cp, const float *d_sf, int
     a.sky_scale = sky_scale;
     a.img = d_img;
     dim3 grid((f->cols + 63) / 64, (f->rows + 3) / 4);
-    if (pow2(hc) && pow2(wc))
-        hipLaunchKernelGGL(k_diff_color<true>, grid, dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_diff_color<false>, grid, dim3(256), 0, s, a);
+    const bool p2 = pow2(hc) && pow2(wc), tame = a.f.tame != 0;
+    void (*kern)(DiffArgs) = p2 ? (tame ? k_diff_color<true, true> : k_diff_color<true, false>)
+                                : (tame ? k_diff_color<false, true> : k_diff_color<false, false>);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
