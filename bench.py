@@ -207,6 +207,8 @@ def bench_diff(ctx, reps):
     """One 1280x720 view of the 2048^2 map, 2048 slices: forward (f32
     planes) and forward + L2 loss + adjoint, the gradient w.r.t. the
     [3, 2048, 2048] colour planes and the shading parameters."""
+    import ctypes
+
     import torch
 
     from futspace_amd import scene
@@ -254,13 +256,55 @@ def bench_diff(ctx, reps):
         step()
         step_separate()
     torch.cuda.synchronize()
-    fwd_kernel_ms = _kernel_ms(ctx, fwd, reps)
-    step_kernel_ms = _kernel_ms(ctx, step, reps)
+
+    # GPU time per call as the plain render's render_kernel_ms measures it:
+    # HIP events on the context stream bracketing reps back-to-back calls of
+    # the C-ABI entry (no torch stream ordering in between)
+    lib, cv, P = ctx.lib, v.to_c(), ctypes.c_void_p
+    sky = ctypes.c_float(scene.SUN_HEIGHT)
+
+    def c_fwd():
+        ctx.check(lib.fsx_render_f32(ctx.ptr, m.ptr, ctypes.byref(cv), P(planes.data_ptr()),
+                                     P(r.sf.data_ptr()), r.hc, r.wc, sky, P(img.data_ptr()),
+                                     P(src.data_ptr())), "fsx_render_f32")
+
+    def c_step():
+        ctx.check(lib.fsx_render_f32_l2_step(
+            ctx.ptr, m.ptr, ctypes.byref(cv), P(planes.data_ptr()), P(r.sf.data_ptr()), r.hc,
+            r.wc, sky, P(target.data_ptr()), 0, P(img.data_ptr()), P(src.data_ptr()),
+            P(gimg.data_ptr()), P(gplanes.data_ptr()), P(gsky.data_ptr()), P(loss.data_ptr())),
+            "fsx_render_f32_l2_step")
+
+    def c_vjp():
+        ctx.check(lib.fsx_render_f32_vjp(
+            ctx.ptr, ctypes.byref(cv), P(planes.data_ptr()), P(r.sf.data_ptr()), r.hc, r.wc,
+            P(src.data_ptr()), P(gimg.data_ptr()), 0, P(gplanes.data_ptr()), P(gsky.data_ptr())),
+            "fsx_render_f32_vjp")
+
+    ext = torch.cuda.ExternalStream(lib.fsx_context_stream(ctx.ptr))
+
+    def region_ms(fn):
+        torch.cuda.synchronize()
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ext)
+        for _ in range(reps):
+            fn()
+        e1.record(ext)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    fwd_kernel_ms = region_ms(c_fwd)
+    step_kernel_ms = region_ms(c_step)
+    vjp_kernel_ms = region_ms(c_vjp)          # the adjoint alone: clear + scatter-add
+    # per-call event pairs (fsx_profile_*) around the torch-ordered calls:
+    # each pair also times its own event packets and the stream ordering
+    fwd_pair_ms = _kernel_ms(ctx, fwd, reps)
+    step_pair_ms = _kernel_ms(ctx, step, reps)
     fwd_ms = wall(fwd)
     step_ms = wall(step)
     separate_ms = wall(step_separate)
-    # the adjoint alone (clear + scatter-add), kernel events
-    vjp_kernel_ms = _kernel_ms(ctx, lambda: r.vjp_raw(planes, src, gimg, v, gplanes, gsky), reps)
     step()
     visible = int((src >= 0).sum())
     return {"workload": "1280x720 view, 2048^2 map, 2048 slices, grad w.r.t. f32 "
@@ -268,6 +312,7 @@ def bench_diff(ctx, reps):
             "fwd_ms": fwd_ms, "fwd_bwd_step_ms": step_ms, "step_over_fwd": step_ms / fwd_ms,
             "fwd_kernel_ms": fwd_kernel_ms, "step_kernel_ms": step_kernel_ms,
             "step_over_fwd_kernels": step_kernel_ms / fwd_kernel_ms,
+            "fwd_kernel_ms_event_pairs": fwd_pair_ms, "step_kernel_ms_event_pairs": step_pair_ms,
             "separate_calls_step_ms": separate_ms, "vjp_kernel_ms": vjp_kernel_ms,
             "loss": float(loss[0]), "terrain_pixels": visible}
 
