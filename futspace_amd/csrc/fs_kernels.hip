@@ -2388,15 +2388,49 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
 }
 
 /* The fused step's two sums: its loss partials into *loss and its sky
- * partials into *gsky (= or +=), each in a fixed order. */
+ * partials into *gsky (= or +=), each in block_sum_partials' order (so
+ * bit-identical to two calls of it), the two arrays' loads, shuffles and
+ * final adds interleaved so that their latencies overlap. */
 __global__ __launch_bounds__(1024) void k_sum_step(const float *__restrict__ loss_part,
                                                    const float *__restrict__ sky_part, int32_t n,
                                                    float *__restrict__ loss, int32_t loss_acc,
                                                    float *__restrict__ gsky, int32_t sky_acc)
 {
-    block_sum_partials<1024>(loss_part, n, loss, loss_acc);
+    constexpr int NT = 1024;
+    __shared__ float wsum[2][NT / 64];
+    float a = 0.0f, b = 0.0f;
+    for (int32_t base = 0; base < n; base += 8 * NT) {
+        float va[8], vb[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int32_t i = base + u * NT + (int32_t)threadIdx.x;
+            va[u] = i < n ? loss_part[i] : 0.0f;
+            vb[u] = i < n ? sky_part[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            a += va[u];
+            b += vb[u];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][threadIdx.x >> 6] = a;
+        wsum[1][threadIdx.x >> 6] = b;
+    }
     __syncthreads();
-    block_sum_partials<1024>(sky_part, n, gsky, sky_acc);
+    if (threadIdx.x == 0 || threadIdx.x == 64) {
+        const int k = threadIdx.x >> 6;
+        float t = 0.0f;
+        for (int w = 0; w < NT / 64; w++)
+            t += wsum[k][w];
+        float *dst = k ? gsky : loss;
+        *dst = (k ? sky_acc : loss_acc) ? *dst + t : t;
+    }
 }
 
 /* L2 loss of the fitting harness: g = 2 * (img - target) and one partial
