@@ -275,6 +275,14 @@ def bench_diff(ctx, reps):
             P(gimg.data_ptr()), P(gplanes.data_ptr()), P(gsky.data_ptr()), P(loss.data_ptr())),
             "fsx_render_f32_l2_step")
 
+    def c_step_fit():
+        # the step as a fitting loop calls it: no image and no g image out
+        ctx.check(lib.fsx_render_f32_l2_step(
+            ctx.ptr, m.ptr, ctypes.byref(cv), P(planes.data_ptr()), P(r.sf.data_ptr()), r.hc,
+            r.wc, sky, P(target.data_ptr()), 0, None, P(src.data_ptr()), None,
+            P(gplanes.data_ptr()), P(gsky.data_ptr()), P(loss.data_ptr())),
+            "fsx_render_f32_l2_step")
+
     def c_vjp():
         ctx.check(lib.fsx_render_f32_vjp(
             ctx.ptr, ctypes.byref(cv), P(planes.data_ptr()), P(r.sf.data_ptr()), r.hc, r.wc,
@@ -297,6 +305,7 @@ def bench_diff(ctx, reps):
 
     fwd_kernel_ms = region_ms(c_fwd)
     step_kernel_ms = region_ms(c_step)
+    step_fit_kernel_ms = region_ms(c_step_fit)
     vjp_kernel_ms = region_ms(c_vjp)          # the adjoint alone: clear + scatter-add
     # per-call event pairs (fsx_profile_*) around the torch-ordered calls:
     # each pair also times its own event packets and the stream ordering
@@ -312,6 +321,7 @@ def bench_diff(ctx, reps):
             "fwd_ms": fwd_ms, "fwd_bwd_step_ms": step_ms, "step_over_fwd": step_ms / fwd_ms,
             "fwd_kernel_ms": fwd_kernel_ms, "step_kernel_ms": step_kernel_ms,
             "step_over_fwd_kernels": step_kernel_ms / fwd_kernel_ms,
+            "step_no_image_out_kernel_ms": step_fit_kernel_ms,
             "fwd_kernel_ms_event_pairs": fwd_pair_ms, "step_kernel_ms_event_pairs": step_pair_ms,
             "separate_calls_step_ms": separate_ms, "vjp_kernel_ms": vjp_kernel_ms,
             "loss": float(loss[0]), "terrain_pixels": visible}
