@@ -178,6 +178,15 @@ __device__ __forceinline__ uint32_t argb_mix(float m1, uint32_t c1, float m2, ui
 
 /* png_height_filtered, fut/render_functions.fut:67-77, from the height
  * quad at (floor y, floor x). */
+/* The maps are device memory.  Their gathers read through pointers of the
+ * global address space: a map pointer the kernel read from memory (a
+ * batch's view records) is generic, and loads through it would be flat
+ * loads, which also count against the LDS wait counter, so every LDS wait
+ * of the tile would wait for the gathers in flight too. */
+#define FS_GLOBAL __attribute__((address_space(1)))
+typedef uint32_t fs_u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t fs_u32x4 __attribute__((ext_vector_type(4)));
+
 template <int HBITS, bool HP2, bool TAME = false>
 __device__ __forceinline__ uint2 height_quad(const fsk_map &m, float x, float y)
 {
@@ -188,9 +197,11 @@ __device__ __forceinline__ uint2 height_quad(const fsk_map &m, float x, float y)
      * wrapped index is always inside the map, whatever x and y are */
     uint32_t t = HP2 ? ((uint32_t)iy << __builtin_ctz((uint32_t)m.hcols)) + (uint32_t)ix
                      : (uint32_t)iy * (uint32_t)m.hcols + (uint32_t)ix;
+    const FS_GLOBAL char *hq = (const FS_GLOBAL char *)m.hq;
     if (HBITS == 8)
-        return make_uint2(*(const uint32_t *)((const char *)m.hq + t * 4u), 0u);
-    return *(const uint2 *)((const char *)m.hq + t * 8u);
+        return make_uint2(*(const FS_GLOBAL uint32_t *)(hq + t * 4u), 0u);
+    const fs_u32x2 v = *(const FS_GLOBAL fs_u32x2 *)(hq + t * 8u);
+    return make_uint2(v.x, v.y);
 }
 
 template <int HBITS>
@@ -535,7 +546,8 @@ __device__ __forceinline__ uint4 color_quad(const fsk_map &m, float x, float y)
      * index fits 32 bits; the byte offset is formed in 64 */
     uint32_t t = CP2 ? ((uint32_t)iy << __builtin_ctz((uint32_t)m.ccols)) + (uint32_t)ix
                      : (uint32_t)iy * (uint32_t)m.ccols + (uint32_t)ix;
-    return ((const uint4 *)m.cq)[t];
+    const fs_u32x4 v = ((const FS_GLOBAL fs_u32x4 *)m.cq)[t];
+    return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 /* The colour of a sample at (x, y) from its texel quad q; with qt (the
