@@ -2144,7 +2144,11 @@ __device__ __forceinline__ bool table_add(uint32_t *keys, float *vals, const uin
 {
     constexpr int kCap = 1 << CAPLOG;
     uint32_t h = (key * 2654435761u) >> (32 - CAPLOG);
-    const bool open = *(const volatile uint32_t *)fill < (uint32_t)FILL_LIMIT;
+    /* a relaxed atomic load, which becomes an LDS read: a volatile read
+     * through this generic pointer was a flat load that waited for every
+     * global atomic of the wave still in flight */
+    const bool open = __hip_atomic_load(fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <
+                      (uint32_t)FILL_LIMIT;
 #pragma unroll 1
     for (int p = 0; open && p < PROBES; p++) {
         uint32_t k = keys[h];
@@ -2271,7 +2275,10 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
     const int32_t H = a.f.rows, W = a.f.cols;
     const bool ok = col < W && r0 < H;
     const size_t plane = (size_t)a.hc * a.wc, fplane = (size_t)H * W;
-    const float *gin = FWD ? a.tgt : a.gimg;
+    /* global-space views of the per-pixel inputs (loads through the
+     * generic pointers came out as flat loads) */
+    const FS_GLOBAL float *gin = (const FS_GLOBAL float *)(FWD ? a.tgt : a.gimg);
+    const FS_GLOBAL int32_t *srcp = (const FS_GLOBAL int32_t *)a.src;
     int32_t src[kGradRows];
     float g[3][kGradRows];     /* FWD: the target until the pixel's g replaces it */
 #pragma unroll
@@ -2279,7 +2286,7 @@ __global__ __launch_bounds__(kGradThreads) void k_diff_grad(DiffArgs a)
         const int32_t r = r0 + j;
         const bool in = ok && r < H;
         const size_t p = (size_t)r * W + col;
-        src[j] = in ? a.src[p] : INT32_MIN;
+        src[j] = in ? srcp[p] : INT32_MIN;
 #pragma unroll
         for (int ch = 0; ch < 3; ch++)
             g[ch][j] = in ? gin[ch * fplane + p] : 0.0f;
